@@ -1,0 +1,187 @@
+#!/usr/bin/env python3
+"""NEMOLite2D time step benchmark on one MI355X.
+
+Workload (default): 4096 x 4096 interior T points, FP64, seeded smooth
+bathymetry (100 m + 50 m relief, sigma 32 cells), open west edge forced by
+1.0 m * sin(2 pi t / 44712 s), dt 20 s, cbfr 1.5e-3, visc 0.1. One step is
+the reference PSy layer's invoke_time_step (continuity, momentum u/v, bc_ssh,
+solid and Flather boundaries, next_sshu/v, the now<-after copies), executed
+by libnemo_hip.so's fused kernel plus the Flather segment kernel, replayed as
+one hipGraph; inputs are resident in HBM before the timed region.
+
+Prints ONE JSON line (rank 0, N = 1):
+  value            interior grid-point updates per second (whole job)
+  roofline         the fused step kernel: algorithmic bytes (212 B per update,
+                   every array of the reference kernel signatures touched once)
+                   over its mean duration from HIP events, against 8 TB/s
+  cpu_baseline     the reference's own OpenMP PSy layer (time_step_omp.cpp,
+                   built into oracle/_ref) on this host's cores, on a bounded
+                   sample of the same workload
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = ("grid-pt updates/s + % HBM peak at 4096²; "
+          "FP32/learned vs plain FP64, same err")
+HBM_PEAK_GBS = 8000.0
+B_ALG_F64 = 212          # bytes per interior grid-point update, FP64
+PROFILE_TRAFFIC = ROOT / "profiles" / "fused_step_f64_4096_traffic.txt"
+
+
+def parse_args():
+    p = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=1000)
+    p.add_argument("--warmup", type=int, default=20)
+    p.add_argument("--n", type=int, default=4096, help="interior points per side")
+    p.add_argument("--kernel-steps", type=int, default=200,
+                   help="steps timed per kernel with HIP events for the roofline")
+    p.add_argument("--cpu-seconds", type=float, default=12.0,
+                   help="target CPU time of the cpu_baseline sample")
+    p.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
+    return p.parse_args()
+
+
+def traffic_from_profile():
+    """HBM bytes per fused-step launch measured with rocprofv3 PMC counters
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), as
+    recorded in profiles/; None when not measured."""
+    if not PROFILE_TRAFFIC.exists():
+        return None
+    for line in PROFILE_TRAFFIC.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def cpu_baseline(n: int, seconds: float) -> dict:
+    """The reference OpenMP PSy layer (oracle/_ref/libref_time_step_omp.so)
+    timed on this host on k steps of the same n^2 workload, k chosen so the
+    sample takes about `seconds`."""
+    import numpy as np  # noqa: F401
+    import oracle
+    from psyclonebench_amd.model import Model, c2_config
+
+    affinity = len(os.sched_getaffinity(0))
+    threads = min(affinity, int(os.environ.get("OMP_NUM_THREADS", affinity)))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    kind = "ref_omp" if oracle.ref_available() else "oracle_omp"
+    m = Model(c2_config(n, 1, seed=0))
+    g = m.grid
+    arrays = m.arrays()
+    step = oracle.step_fn(kind)
+    t0 = time.perf_counter()
+    step(arrays, 1, g.bounds, g.W, m.scalars)
+    one = time.perf_counter() - t0
+    k = max(2, min(200, int(seconds / max(one, 1e-6))))
+    t0 = time.perf_counter()
+    for istp in range(2, 2 + k):
+        step(arrays, istp, g.bounds, g.W, m.scalars)
+    dt = time.perf_counter() - t0
+    pts = (g.M - 2) * (g.N - 2)
+    return {
+        "value": pts * k / dt,
+        "unit": "grid-pt updates/s",
+        "cores": threads,
+        "kind": "reference" if kind == "ref_omp" else "port",
+        "sample": (f"{k} steps of the {n}x{n} FP64 workload after 1 untimed step; "
+                   f"reference manual_versions/psykal_cpp/time_step_omp.cpp, "
+                   f"OpenMP {threads} threads, g++ -O3 -mtune=native -ffp-contract=off"),
+        "seconds": dt,
+    }
+
+
+def main():
+    a = parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+              "(the step would shard by 2-D domain decomposition with depth-1 halos; "
+              "not built)", file=sys.stderr)
+        sys.exit(2)
+
+    from psyclonebench_amd import _native
+    from psyclonebench_amd.model import Model, c2_config
+    from psyclonebench_amd.psy import TimeStepper
+
+    lib = _native.hip_lib()
+    ndev, info = _native.device_info()
+    if ndev < 1:
+        raise SystemExit(f"bench.py: no GPU visible ({info})")
+
+    cfg = c2_config(a.n, a.warmup + a.steps, seed=0)
+    model = Model(cfg)
+    g = model.grid
+    pts = (g.M - 2) * (g.N - 2)
+    st = TimeStepper(model, mode=2)
+    # first call uploads, pairs the ping-pong buffers; then warmup (captures
+    # the graph of the timed length once, untimed)
+    st.run(1 + a.warmup)
+    st.synchronize()
+    st.run(a.steps)             # capture + first replay of a K-step graph
+    st.synchronize()
+    t0 = time.perf_counter()
+    st.run(a.steps)
+    st.synchronize()
+    wall = time.perf_counter() - t0
+    updates_per_s = pts * a.steps / wall
+
+    # roofline: mean duration of the fused kernel from HIP events
+    lib.nemo_hip_set_kernel_timing(1)
+    st.run(a.kernel_steps)
+    kernel_ms = lib.nemo_hip_fused_kernel_ms()
+    lib.nemo_hip_set_kernel_timing(0)
+    st.release()
+    bytes_alg = B_ALG_F64 * pts
+    achieved = bytes_alg / (kernel_ms * 1e-3) / 1e9
+    traffic = traffic_from_profile()
+
+    out = {
+        "metric": METRIC,
+        "value": updates_per_s,
+        "unit": "grid-pt updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": wall * 1e3 / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"NEMOLite2D FP64 step, {a.n}x{a.n} interior T points, seeded "
+                         "bathymetry (seed 0), open west edge with 1 m tide, fused "
+                         "kernel + Flather segments, hipGraph replay"),
+            "interior": [g.M - 2, g.N - 2],
+            "alloc": [g.W, g.H],
+            "device": info,
+            "hbm_frac_whole_step": B_ALG_F64 * updates_per_s / (HBM_PEAK_GBS * 1e9),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "kernel": "k_fused_step",
+            "kernel_ms": kernel_ms,
+            "bytes_per_launch_alg": bytes_alg,
+        },
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(a.n, a.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,156 @@
+/*
+ * libnemo_hip.so -- NEMOLite2D time step on one AMD Instinct MI355X (gfx950)
+ * behind the C ABI of the reference's manual PSy layer.
+ *
+ * Plain C: pointers, ints and doubles only. Errors print the failing call
+ * and exit(1), the reference's convention (psykal_c_opencl/opencl_utils.c:
+ * 218-226, check_status).
+ *
+ * Arrays follow the dl_esm_inf layout: each is `width` doubles per row,
+ * point (ji, jj) at jj*width + ji (0-based); tmask is int32 with the same
+ * pitch. Bounds are the 0-based internal T region (internal%xstart-1 ...).
+ *
+ * Entry points and the reference interface each one replaces:
+ *
+ *   c_invoke_time_step           device-resident PSy layer,
+ *                                manual_versions/psykal_c_opencl/time_step.c:
+ *                                100-169 (Fortran interface
+ *                                psykal_c_opencl/time_step_mod.f90:11-70)
+ *   c_read_from_device           psykal_c_opencl/time_step.c:788-797
+ *   c_write_to_device            psykal_c_opencl/time_step.c:801-810
+ *   c_invoke_time_step_host      host-pointer PSy layer,
+ *                                manual_versions/psykal_cpp/time_step.cpp:
+ *                                10-51 (libnemo_hip_cpp.so re-exports it
+ *                                under the reference name c_invoke_time_step
+ *                                for the psykal_cpp/time_step_mod.f90 shim)
+ *   c_invoke_time_steps          no reference counterpart: n steps of
+ *                                c_invoke_time_step replayed as one hipGraph
+ *   nemo_hip_*                   no reference counterpart: configuration,
+ *                                timing and teardown
+ */
+#ifndef NEMOLITE2D_HIP_H
+#define NEMOLITE2D_HIP_H
+
+#include <stdbool.h>
+
+#ifndef NEMO_API
+#define NEMO_API __attribute__((visibility("default")))
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* The 25 host arrays, in the reference's argument order
+ * (psykal_cpp/time_step.cpp:12-37). */
+#define NEMO_HOST_ARRAYS                                                      \
+    double *ssha_t, double *sshn_t, double *sshn_u, double *sshn_v,           \
+    double *hu, double *hv, double *un, double *vn, double *ua, double *ht,   \
+    double *ssha_u, double *va, double *ssha_v, int *tmask, double *area_t,   \
+    double *area_u, double *area_v, double *dx_u, double *dx_v, double *dx_t, \
+    double *dy_u, double *dy_v, double *dy_t, double *gphiu, double *gphiv
+
+/* The 25 device-handle slots, passed by reference in the same order
+ * (psykal_c_opencl/time_step.c:138-162, cl_mem* there). */
+#define NEMO_DEVICE_SLOTS                                                     \
+    void **ssha_t_device, void **sshn_t_device, void **sshn_u_device,         \
+    void **sshn_v_device, void **hu_device, void **hv_device,                 \
+    void **un_device, void **vn_device, void **ua_device, void **ht_device,   \
+    void **ssha_u_device, void **va_device, void **ssha_v_device,             \
+    void **tmask_device, void **area_t_device, void **area_u_device,          \
+    void **area_v_device, void **dx_u_device, void **dx_v_device,             \
+    void **dx_t_device, void **dy_u_device, void **dy_v_device,               \
+    void **dy_t_device, void **gphiu_device, void **gphiv_device
+
+/*
+ * Device-resident step (psykal_c_opencl/time_step.c:100-169).
+ * First call for a given set of slots: allocates device buffers, uploads
+ * all 25 host arrays (total_size elements each) and stores the handles in
+ * the slots. Every call: one time step on the device, asynchronous on the
+ * library's stream; the host arrays are not read again and not updated --
+ * fetch results with c_read_from_device(*slot, host, ...).
+ * After a call, the slots of each after/now pair (ssha_t/sshn_t, ua/un,
+ * va/vn) hold one handle: the reference copies after->now at the end of the
+ * step, here the pair shares one buffer instead. The handles change from
+ * call to call (ping-pong); always read the slots after the call.
+ */
+#ifndef NEMO_PSYKAL_CPP_ONLY
+NEMO_API void c_invoke_time_step(NEMO_HOST_ARRAYS, NEMO_DEVICE_SLOTS, int istep,
+                        int internal_xstart, int internal_xstop,
+                        int internal_ystart, int internal_ystop, int width,
+                        int total_size, double rdt, double cbfr, double visc,
+                        double omega, double d2r, double g);
+#endif
+
+/* nsteps consecutive steps istep, istep+1, ... as one replay of a captured
+ * hipGraph (captured on first use of a given nsteps). Same arguments and
+ * slot semantics as c_invoke_time_step. */
+NEMO_API void c_invoke_time_steps(int nsteps, NEMO_HOST_ARRAYS, NEMO_DEVICE_SLOTS,
+                         int istep, int internal_xstart, int internal_xstop,
+                         int internal_ystart, int internal_ystop, int width,
+                         int total_size, double rdt, double cbfr, double visc,
+                         double omega, double d2r, double g);
+
+/* Blocking copy of nx*ny doubles from device handle `from` to host `to`,
+ * after all queued steps have finished. As in the reference, the copy is
+ * the contiguous block from the start of the buffer; startx/starty and
+ * blocking are accepted and ignored (the copy is always blocking). */
+NEMO_API void c_read_from_device(void *from, void *to, int startx, int starty, int nx,
+                        int ny, bool blocking);
+
+/* Blocking copy of nx*ny doubles from host `from` to device handle `to`
+ * (the reference body reads instead of writing; this writes). */
+NEMO_API void c_write_to_device(void *from, void *to, int startx, int starty, int nx,
+                       int ny, bool blocking);
+
+/* Host-pointer step (psykal_cpp/time_step.cpp:10-51): uploads the arrays,
+ * runs the unfused exact kernel sequence, downloads ssha_t, sshn_t, sshn_u,
+ * sshn_v, un, vn, ua, va, and returns with the host arrays updated. */
+NEMO_API void c_invoke_time_step_host(NEMO_HOST_ARRAYS, int istep, int internal_xstart,
+                             int internal_xstop, int internal_ystart,
+                             int internal_ystop, int width, double rdt,
+                             double cbfr, double visc, double omega,
+                             double d2r, double g);
+
+/* ---- configuration, timing, teardown (no reference counterpart) ------- */
+
+/* Tidal forcing of bc_ssh: ssha = amp * sin(omega_tide * istep * rdt).
+ * Default: the reference literals amp = 0.2, omega_tide = 2*3.14159/
+ * (12.42*3600) (boundary_conditions_kern.c:185-186). */
+NEMO_API void nemo_hip_set_tide(double amp, double omega_tide);
+
+/* Execution form for contexts created afterwards:
+ *  0 auto    fused step when the mask allows it (no wet cell on the last
+ *            T row, which the fused form does not copy), unfused otherwise
+ *  1 unfused one kernel per reference sweep, in place (exact for any input)
+ *  2 fused   fused step; exits with an error when the mask does not allow it
+ */
+NEMO_API void nemo_hip_set_mode(int mode);
+
+/* Form in use by the context whose first slot is ssha_t_device:
+ * 1 unfused, 2 fused, 0 no such context. */
+NEMO_API int nemo_hip_context_mode(void **ssha_t_device);
+
+/* Waits for all queued work of the library. */
+NEMO_API void nemo_hip_synchronize(void);
+
+/* Average device time in ms of one fused step kernel over the fused steps
+ * of the last c_invoke_time_steps call made with timing on, measured with
+ * HIP events on the library stream around each fused launch (timing on
+ * runs the steps as plain launches, not as a graph). -1 if none. */
+NEMO_API void nemo_hip_set_kernel_timing(int on);
+NEMO_API double nemo_hip_fused_kernel_ms(void);
+
+/* Releases the device buffers, graphs and lists of one context (identified
+ * by its ssha_t slot), or of all contexts. Slots become dangling. */
+NEMO_API void nemo_hip_release(void **ssha_t_device);
+NEMO_API void nemo_hip_release_all(void);
+
+/* Device name and HBM size into buf; returns the device count (0 when no
+ * GPU is visible; never exits). */
+NEMO_API int nemo_hip_device_info(char *buf, int buflen);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

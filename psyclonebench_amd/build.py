@@ -1,0 +1,88 @@
+"""Builds the native libraries of psyclonebench_amd in-tree.
+
+* ``libnemo_setup.so``  host model set-up (g++, OpenMP)
+* ``libnemo_hip.so``    HIP kernels + C-ABI PSy layer for gfx950 (hipcc)
+* ``libnemo_hip_cpp.so`` re-exports the host-pointer step under the
+  reference's psykal_cpp name ``c_invoke_time_step``
+
+Every build runs the compilers again (no staleness check): callers rely on
+``build()`` to pick up source edits.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+INCLUDE = ROOT / "include"
+
+ARCH = os.environ.get("NEMO_OFFLOAD_ARCH", "gfx950")
+
+# -ffp-contract=off: no FMA contraction, so FP64 rounds like the reference C.
+# The packed-FP32 feature is disabled for every HIP translation unit.
+HIPCC_FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-std=c++17",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-fPIC",
+    "-fvisibility=hidden",
+    "-Wall",
+    "-Wno-unused-command-line-argument",
+]
+
+HIP_SOURCES = ["nemo_step.hip", "nemo_psy.cpp"]
+
+
+def _hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and Path(cand).exists():
+            return cand
+    raise RuntimeError("hipcc not found (ROCm is required to build libnemo_hip.so)")
+
+
+def _run(cmd: list[str], verbose: bool) -> None:
+    if verbose:
+        print("+", " ".join(cmd), flush=True)
+    res = subprocess.run(cmd, cwd=ROOT, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError(f"build step failed: {cmd[0]} (exit {res.returncode})")
+    noisy = [l for l in (res.stderr or "").splitlines()
+             if "packed-fp32-ops" not in l and l.strip()]
+    if verbose and noisy:
+        sys.stderr.write("\n".join(noisy) + "\n")
+
+
+def build_setup(verbose: bool = True) -> Path:
+    out = PKG / "libnemo_setup.so"
+    _run(["g++", "-O2", "-std=c++17", "-fopenmp", "-fPIC", "-shared",
+          f"-I{INCLUDE}", str(CSRC / "nemo_setup.cpp"), "-o", str(out)], verbose)
+    return out
+
+
+def build_hip(verbose: bool = True) -> Path:
+    out = PKG / "libnemo_hip.so"
+    srcs = [str(CSRC / s) for s in HIP_SOURCES]
+    _run([_hipcc(), *HIPCC_FLAGS, "-shared", f"-I{INCLUDE}", f"-I{CSRC}",
+          *srcs, "-Wl,-Bsymbolic", "-o", str(out)], verbose)
+    cpp = PKG / "libnemo_hip_cpp.so"
+    _run(["gcc", "-O2", "-fPIC", "-shared", f"-I{INCLUDE}",
+          str(CSRC / "psykal_cpp_abi.c"), "-o", str(cpp),
+          f"-L{PKG}", "-lnemo_hip", "-Wl,-rpath,$ORIGIN"], verbose)
+    return out
+
+
+def build(verbose: bool = True) -> None:
+    build_setup(verbose)
+    build_hip(verbose)
+
+
+if __name__ == "__main__":
+    build(verbose=True)
