@@ -21,6 +21,7 @@ Prints ONE JSON line (rank 0, N = 1):
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -45,6 +46,8 @@ def parse_args():
     p.add_argument("--n", type=int, default=4096, help="interior points per side")
     p.add_argument("--kernel-steps", type=int, default=200,
                    help="steps timed per kernel with HIP events for the roofline")
+    p.add_argument("--tune-steps", type=int, default=20,
+                   help="steps per fused-kernel shape in the autotune pass")
     p.add_argument("--cpu-seconds", type=float, default=12.0,
                    help="target CPU time of the cpu_baseline sample")
     p.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
@@ -126,6 +129,18 @@ def main():
     # the graph of the timed length once, untimed)
     st.run(1 + a.warmup)
     st.synchronize()
+    # autotune the fused kernel shape on this workload (HIP-event timed)
+    tune = {}
+    for v in range(-1, lib.nemo_hip_fused_variant_count()):
+        lib.nemo_hip_set_fused_variant(v)
+        lib.nemo_hip_set_kernel_timing(1)
+        st.run(a.tune_steps)
+        tune[v] = lib.nemo_hip_fused_kernel_ms()
+        lib.nemo_hip_set_kernel_timing(0)
+    best = min(tune, key=tune.get)
+    lib.nemo_hip_set_fused_variant(best)
+    bw, rows = C.c_int(), C.c_int()
+    lib.nemo_hip_fused_variant_shape(best, C.byref(bw), C.byref(rows))
     st.run(a.steps)             # capture + first replay of a K-step graph
     st.synchronize()
     t0 = time.perf_counter()
@@ -165,6 +180,9 @@ def main():
             "alloc": [g.W, g.H],
             "device": info,
             "hbm_frac_whole_step": B_ALG_F64 * updates_per_s / (HBM_PEAK_GBS * 1e9),
+            "fused_variant": best,
+            "fused_shape": [bw.value, rows.value],
+            "variant_kernel_ms": {str(k): round(v, 4) for k, v in tune.items()},
         },
         "roofline": {
             "bound": "hbm",

@@ -141,6 +141,17 @@ NEMO_API void nemo_hip_synchronize(void);
 NEMO_API void nemo_hip_set_kernel_timing(int on);
 NEMO_API double nemo_hip_fused_kernel_ms(void);
 
+/* Kernel shape of the fused form: -1 = 64x4-cell tiles with an LDS ssh
+ * tile; 0 .. nemo_hip_fused_variant_count()-1 = row-march shapes (block
+ * width x rows per band). All shapes compute bitwise the same result;
+ * default 0. Applies to the next launches of every context. */
+NEMO_API int nemo_hip_fused_variant_count(void);
+NEMO_API void nemo_hip_set_fused_variant(int variant);
+NEMO_API int nemo_hip_fused_variant(void);
+/* Block width (columns) and rows per block of a variant. */
+NEMO_API void nemo_hip_fused_variant_shape(int variant, int *block_width,
+                                           int *rows);
+
 /* Releases the device buffers, graphs and lists of one context (identified
  * by its ssha_t slot), or of all contexts. Slots become dangling. */
 NEMO_API void nemo_hip_release(void **ssha_t_device);

@@ -37,7 +37,7 @@ HIPCC_FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 
-HIP_SOURCES = ["nemo_step.hip", "nemo_psy.cpp"]
+HIP_SOURCES = ["nemo_step.hip", "nemo_march.hip", "nemo_psy.cpp"]
 
 
 def _hipcc() -> str:

@@ -6,6 +6,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -36,6 +37,7 @@ struct Physics {
 double g_amp_tide = 0.2;
 double g_omega_tide = 2.0 * 3.14159 / (12.42 * 3600.0);
 int g_mode = 0;
+int g_fused_variant = 0;  // -1: tile kernel; >= 0: row-march variant
 bool g_kernel_timing = false;
 double g_fused_ms = -1.0;
 
@@ -163,7 +165,7 @@ struct Context {
     double *d_tide = nullptr;
     double *h_tide = nullptr;  // pinned
     int tide_cap = 0;
-    std::map<std::pair<int, int>, hipGraphExec_t> graphs;
+    std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;  // n, parity, variant
     std::vector<hipEvent_t> ev;
 
     double *d(Arr a) const { return static_cast<double *>(buf[a]); }
@@ -350,10 +352,14 @@ void launch_fused_pair(Context &C, int par, hipStream_t s, const double *tide_ta
 {
     const int q = 1 - par;
     const StepOut<double> o{C.T(q), C.U(q), C.V(q), C.SU(q), C.SV(q)};
+    const Bounds bd{C.G.xs, C.G.xe, C.G.ys, C.G.ye};
     if (e0) NEMO_HIP_CHECK(hipEventRecord(e0, s));
-    launch_fused_step(s, C.inputs(par), C.scalars(), o, C.G.W,
-                      Bounds{C.G.xs, C.G.xe, C.G.ys, C.G.ye}, tide_tab, tide_k,
-                      tide_val);
+    if (g_fused_variant < 0)
+        launch_fused_step(s, C.inputs(par), C.scalars(), o, C.G.W, bd, tide_tab,
+                          tide_k, tide_val);
+    else
+        launch_fused_march(s, g_fused_variant, C.inputs(par), C.scalars(), o,
+                           C.G.W, bd, tide_tab, tide_k, tide_val);
     if (e1) NEMO_HIP_CHECK(hipEventRecord(e1, s));
     launch_flather(s, C.fl_u.view(), C.fl_v.view(), C.U(q), C.V(q), C.SU(par),
                    C.SV(par));
@@ -393,7 +399,7 @@ void ensure_tide_table(Context &C, int n)
 
 hipGraphExec_t fused_graph(Context &C, int n, int par, hipStream_t s)
 {
-    auto key = std::make_pair(n, par);
+    auto key = std::make_tuple(n, par, g_fused_variant);
     auto it = C.graphs.find(key);
     if (it != C.graphs.end()) return it->second;
     hipGraph_t graph;
@@ -616,6 +622,27 @@ int nemo_hip_context_mode(void **ssha_t_device)
 void nemo_hip_synchronize(void) { NEMO_HIP_CHECK(hipStreamSynchronize(lib_stream())); }
 
 void nemo_hip_set_kernel_timing(int on) { g_kernel_timing = on != 0; }
+
+int nemo_hip_fused_variant_count(void) { return march_variant_count(); }
+
+void nemo_hip_set_fused_variant(int variant)
+{
+    if (variant < -1 || variant >= march_variant_count())
+        NEMO_FATAL("unknown fused variant %d", variant);
+    g_fused_variant = variant;
+}
+
+int nemo_hip_fused_variant(void) { return g_fused_variant; }
+
+void nemo_hip_fused_variant_shape(int variant, int *block_width, int *rows)
+{
+    if (variant == -1) {
+        *block_width = 64;
+        *rows = 4;
+        return;
+    }
+    march_variant_shape(variant, block_width, rows);
+}
 
 double nemo_hip_fused_kernel_ms(void) { return g_fused_ms; }
 

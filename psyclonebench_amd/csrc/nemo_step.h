@@ -76,6 +76,16 @@ void launch_fused_step(hipStream_t s, const StepIn<double> &f,
                        int W, Bounds b, const double *tide_tab, int tide_k,
                        double tide_val);
 
+// Row-march form of the fused step (nemo_march.hip): same inputs, outputs
+// and arithmetic as launch_fused_step; `variant` picks the block width and
+// rows per band (0 .. march_variant_count()-1).
+void launch_fused_march(hipStream_t s, int variant, const StepIn<double> &f,
+                        const StepScalars<double> &p, const StepOut<double> &o,
+                        int W, Bounds b, const double *tide_tab, int tide_k,
+                        double tide_val);
+int march_variant_count();
+void march_variant_shape(int variant, int *mx, int *rows);
+
 // Small helpers.
 void launch_fill_int(hipStream_t s, int *p, int n, int value);
 

@@ -1,0 +1,63 @@
+"""Fixed workload for rocprofv3 runs (kernel trace and PMC counters).
+
+    python -m psyclonebench_amd.profile_run --n 4096 --fused 10 --unfused 2
+
+Runs `--fused` fused steps of the 4096^2 FP64 workload as plain launches
+(one k_fused_step + one k_flather dispatch per step) and `--unfused` steps of
+the unfused sequence, whose k_copy3 dispatch is a plain copy of known size
+(3 arrays read + 3 written over the copy region) used to calibrate the
+FETCH_SIZE / WRITE_SIZE counters. Prints the dispatch geometry and the
+algorithmic byte counts the profiles are compared with.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+
+from . import _native
+from .model import Model, c2_config
+from .psy import TimeStepper
+
+B_ALG_F64 = 212
+
+
+def main() -> None:
+    p = argparse.ArgumentParser()
+    p.add_argument("--n", type=int, default=4096)
+    p.add_argument("--fused", type=int, default=10)
+    p.add_argument("--unfused", type=int, default=2)
+    p.add_argument("--variant", type=int, default=0,
+                   help="fused kernel shape (-1 tile, 0.. row march)")
+    a = p.parse_args()
+    lib = _native.hip_lib()
+    lib.nemo_hip_set_fused_variant(a.variant)
+    cfg = c2_config(a.n, 1, seed=0)
+    out = {"n": a.n, "variant": a.variant}
+    if a.fused:
+        m = Model(cfg)
+        g = m.grid
+        st = TimeStepper(m, mode=2)
+        lib.nemo_hip_set_kernel_timing(1)      # plain launches, event-timed
+        st.run(a.fused + 1)
+        out["fused_kernel_ms"] = lib.nemo_hip_fused_kernel_ms()
+        lib.nemo_hip_set_kernel_timing(0)
+        st.release()
+        pts = (g.M - 2) * (g.N - 2)
+        out["fused_alg_bytes_per_launch"] = B_ALG_F64 * pts
+    if a.unfused:
+        m = Model(cfg)
+        g = m.grid
+        st = TimeStepper(m, mode=1)
+        for _ in range(a.unfused):
+            st.step()
+        st.synchronize()
+        st.release()
+        rows = g.ye - g.ys + 2          # ys-1 .. ye
+        cols = g.xe - g.xs + 3          # xs-1 .. xe+1
+        out["copy3_bytes_read"] = 3 * 8 * rows * cols
+        out["copy3_bytes_written"] = 3 * 8 * rows * cols
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,84 @@
+"""Summarises rocprofv3 result databases (kernel trace and PMC passes) into
+the short text kept under profiles/.
+
+    python -m psyclonebench_amd.profile_summary --trace T.db --fetch F.db \
+        --write W.db --kernel k_fused_march --alg-bytes 3556769792
+
+Calibration: k_copy3 (the unfused path's now<-after copy) reads and writes a
+known number of bytes; FETCH_SIZE and WRITE_SIZE of every kernel are scaled
+by (known bytes / counter value) of that copy kernel.
+"""
+from __future__ import annotations
+
+import argparse
+import sqlite3
+
+
+def _short(name: str) -> str:
+    base = name.replace("(anonymous namespace)::", "")
+    if base.startswith("void "):
+        base = base[5:]
+    return base.split("(")[0].split("::")[-1]
+
+
+def kernel_stats(db: str) -> list[tuple[str, int, float, float]]:
+    """(name, calls, total_us, average_us) per kernel from top_kernels."""
+    c = sqlite3.connect(db)
+    rows = c.execute("select name, total_calls, total_duration, average from top_kernels")
+    return [(r[0], int(r[1]), float(r[2]), float(r[3])) for r in rows]
+
+
+def counter_means(db: str, counter: str) -> dict[str, tuple[int, float, float]]:
+    c = sqlite3.connect(db)
+    q = ("select kernel_name, count(*), avg(value), avg(duration) from counters_collection "
+         "where counter_name=? group by kernel_name")
+    return {r[0]: (int(r[1]), float(r[2]) * 1024.0, float(r[3])) for r in c.execute(q, (counter,))}
+
+
+def main() -> None:
+    p = argparse.ArgumentParser()
+    p.add_argument("--trace", required=True)
+    p.add_argument("--fetch", required=True)
+    p.add_argument("--write", required=True)
+    p.add_argument("--kernel", required=True, help="substring of the kernel to report")
+    p.add_argument("--alg-bytes", type=float, required=True,
+                   help="algorithmic bytes per launch of that kernel")
+    p.add_argument("--copy-bytes", type=float, required=True,
+                   help="bytes k_copy3 reads (= writes) per launch")
+    a = p.parse_args()
+
+    print("## kernel trace (rocprofv3 --kernel-trace --stats)")
+    print(f"{'kernel':<40} {'calls':>6} {'avg_us':>10} {'total_ms':>10}")
+    for name, calls, total, avg in kernel_stats(a.trace):
+        print(f"{_short(name)[:40]:<40} {calls:>6} {avg:>10.2f} {total / 1e3:>10.3f}")
+
+    fetch = counter_means(a.fetch, "FETCH_SIZE")
+    write = counter_means(a.write, "WRITE_SIZE")
+    copy_f = [v for k, v in fetch.items() if "k_copy3" in k][0][1]
+    copy_w = [v for k, v in write.items() if "k_copy3" in k][0][1]
+    kf = a.copy_bytes / copy_f
+    kw = a.copy_bytes / copy_w
+    print()
+    print("## HBM traffic (rocprofv3 --pmc FETCH_SIZE, --pmc WRITE_SIZE)")
+    print(f"calibration on k_copy3 ({a.copy_bytes:.0f} B read, same written): "
+          f"FETCH_SIZE x {kf:.3f}, WRITE_SIZE x {kw:.3f}")
+    print(f"{'kernel':<40} {'read_MB':>10} {'write_MB':>10} {'dur_us':>10}")
+    for k in sorted(fetch):
+        f = fetch[k][1] * kf
+        w = write.get(k, (0, 0.0, 0.0))[1] * kw
+        print(f"{_short(k)[:40]:<40} {f / 1e6:>10.1f} {w / 1e6:>10.1f} {fetch[k][2] / 1e3:>10.1f}")
+    key = [k for k in fetch if a.kernel in k][0]
+    traffic = fetch[key][1] * kf + write[key][1] * kw
+    dur_ns = 1e3 * [avg for name, _, _, avg in kernel_stats(a.trace) if a.kernel in name][0]
+    print()
+    print(f"kernel: {_short(key)}")
+    print(f"algorithmic_bytes_per_launch: {a.alg_bytes:.0f}")
+    print(f"traffic_bytes_per_launch: {traffic:.0f}")
+    print(f"traffic_over_algorithmic: {traffic / a.alg_bytes:.3f}")
+    print(f"avg_duration_us (kernel trace): {dur_ns / 1e3:.2f}")
+    print(f"achieved_algorithmic_GBps: {a.alg_bytes / dur_ns:.1f}")
+    print(f"achieved_traffic_GBps: {traffic / dur_ns:.1f}")
+
+
+if __name__ == "__main__":
+    main()

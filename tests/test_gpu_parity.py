@@ -32,8 +32,15 @@ MODE_AUTO, MODE_UNFUSED, MODE_FUSED = 0, 1, 2
 @pytest.fixture(autouse=True)
 def _release_contexts():
     yield
-    _native.hip_lib().nemo_hip_release_all()
-    _native.hip_lib().nemo_hip_set_kernel_timing(0)
+    lib = _native.hip_lib()
+    lib.nemo_hip_release_all()
+    lib.nemo_hip_set_kernel_timing(0)
+    lib.nemo_hip_set_fused_variant(0)
+
+
+def _variants():
+    # -1 = tile kernel, 0.. = row-march shapes (count fixed by the library)
+    return list(range(-1, 7))
 
 
 def test_gpu_visible():
@@ -68,6 +75,30 @@ def test_fused_graph_matches_reference_fixtures(name):
     for cp, fields in out.items():
         for n in c["fields"]:
             assert_bitwise(fields[n], fx[f"{n}@{cp}"], f"{name} {n}@{cp}")
+
+
+def test_variant_count_matches_tests():
+    assert _native.hip_lib().nemo_hip_fused_variant_count() == len(_variants()) - 1
+
+
+@pytest.mark.parametrize("variant", _variants())
+@pytest.mark.parametrize("name", ["ref_s32_east", "ref_s64_southeast", "c1mask_s128_west",
+                                  "bathy_s64_west"])
+def test_every_fused_variant_matches_reference_fixtures(name, variant):
+    print("case", name, "variant", variant, flush=True)
+    _native.hip_lib().nemo_hip_set_fused_variant(variant)
+    c, out, mode = run_case_gpu(name, MODE_FUSED, use_graph=True)
+    fx = load_npz(f"steps_{name}.npz")
+    for cp, fields in out.items():
+        for n in c["fields"]:
+            assert_bitwise(fields[n], fx[f"{n}@{cp}"], f"{name} v{variant} {n}@{cp}")
+
+
+@pytest.mark.parametrize("variant", _variants())
+@pytest.mark.parametrize("mask", ["channels", "east_open", "checker"])
+def test_every_fused_variant_adversarial(mask, variant):
+    _native.hip_lib().nemo_hip_set_fused_variant(variant)
+    test_fused_one_step_adversarial(mask)
 
 
 @pytest.mark.parametrize("name", ["ref_s32_south", "ref_s64_southeast", "bathy_s64_west"])
@@ -277,3 +308,25 @@ def test_full_size_4096_fused_equals_unfused():
     for n in ("sshn_t", "un", "vn", "sshn_u", "sshn_v"):
         x, y = sa.fetch(n), sb.fetch(n)
         assert_bitwise(x[:a.grid.N - 1], y[:a.grid.N - 1], n)
+
+
+@pytest.mark.slow
+def test_full_size_4096_every_variant_equal():
+    """Every fused kernel shape gives the same bits at full size (odd
+    interior 4093 so that no shape divides the extent)."""
+    cfg = c2_config(4093, 1, seed=2)
+    ref = None
+    for v in _variants():
+        print("variant", v, flush=True)
+        _native.hip_lib().nemo_hip_set_fused_variant(v)
+        m = Model(cfg)
+        st = TimeStepper(m, MODE_FUSED)
+        st.run(6)
+        out = {n: st.fetch(n)[:m.grid.N - 1] for n in ("sshn_t", "un", "vn", "sshn_u",
+                                                      "sshn_v")}
+        st.release()
+        if ref is None:
+            ref = out
+        else:
+            for n in out:
+                assert_bitwise(out[n], ref[n], f"variant {v} {n}")
