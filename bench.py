@@ -139,8 +139,8 @@ def main():
         lib.nemo_hip_set_kernel_timing(0)
     best = min(tune, key=tune.get)
     lib.nemo_hip_set_fused_variant(best)
-    bw, rows = C.c_int(), C.c_int()
-    lib.nemo_hip_fused_variant_shape(best, C.byref(bw), C.byref(rows))
+    bw, rows, pf = C.c_int(), C.c_int(), C.c_int()
+    lib.nemo_hip_fused_variant_shape(best, C.byref(bw), C.byref(rows), C.byref(pf))
     st.run(a.steps)             # capture + first replay of a K-step graph
     st.synchronize()
     t0 = time.perf_counter()
@@ -181,7 +181,8 @@ def main():
             "device": info,
             "hbm_frac_whole_step": B_ALG_F64 * updates_per_s / (HBM_PEAK_GBS * 1e9),
             "fused_variant": best,
-            "fused_shape": [bw.value, rows.value],
+            "fused_shape": {"block_columns": bw.value, "rows": rows.value,
+                            "prefetch": bool(pf.value)},
             "variant_kernel_ms": {str(k): round(v, 4) for k, v in tune.items()},
         },
         "roofline": {

@@ -148,9 +148,10 @@ NEMO_API double nemo_hip_fused_kernel_ms(void);
 NEMO_API int nemo_hip_fused_variant_count(void);
 NEMO_API void nemo_hip_set_fused_variant(int variant);
 NEMO_API int nemo_hip_fused_variant(void);
-/* Block width (columns) and rows per block of a variant. */
+/* Block width (columns), rows per block, and whether the row loads are
+ * issued one row ahead (1) of a variant. */
 NEMO_API void nemo_hip_fused_variant_shape(int variant, int *block_width,
-                                           int *rows);
+                                           int *rows, int *prefetch);
 
 /* Releases the device buffers, graphs and lists of one context (identified
  * by its ssha_t slot), or of all contexts. Slots become dangling. */

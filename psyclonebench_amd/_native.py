@@ -84,7 +84,7 @@ def hip_lib() -> C.CDLL:
         lib.nemo_hip_set_fused_variant.argtypes = [C.c_int]
         lib.nemo_hip_fused_variant.restype = C.c_int
         lib.nemo_hip_fused_variant_count.restype = C.c_int
-        lib.nemo_hip_fused_variant_shape.argtypes = [C.c_int, _ip, _ip]
+        lib.nemo_hip_fused_variant_shape.argtypes = [C.c_int, _ip, _ip, _ip]
         lib.nemo_hip_release.argtypes = [_vpp]
         lib.nemo_hip_device_info.argtypes = [C.c_char_p, C.c_int]
         lib.nemo_hip_device_info.restype = C.c_int

@@ -33,7 +33,77 @@ __device__ __forceinline__ bool in_rect(int i, int j, int x0, int x1, int y0,
     return i >= x0 && i <= x1 && j >= y0 && j <= y1;
 }
 
-template <int MX, int R>
+// Everything one lane loads for row r: its own column i, the neighbours
+// i-1 / i+1 (i+2 for one tmask entry), the tmask of row r+1, and the values
+// of row r-1 that only the point itself reads.
+template <typename T>
+struct RowRaw {
+    T un, vn, hu, su, hv, sv, e1v, e2u, ht, st, e1t, e2t, at;      // (i, r)
+    T un_w, vn_w, hu_w, su_w, hv_w, sv_w, e1v_w, e2u_w;              // (i-1, r)
+    T un_e, vn_e, hu_e, su_e, hv_e, sv_e, e1v_e, ht_e, st_e, e1t_e, e2t_e,
+        at_e;                                                        // (i+1, r)
+    int tm_w, tm_ee;                                                 // (i-1|i+2, r)
+    int tm_n, tm_ne;                                                 // (i|i+1, r+1)
+    T dxu, dyv, fcu, fcv, aru, arv, sau, sav;                        // (i, r-1)
+};
+
+template <typename T>
+__device__ __forceinline__ void load_raw(RowRaw<T> &q, const StepIn<T> &f,
+                                         long r, long rn, long rp, int i, int cw,
+                                         int ce, int cee)
+{
+    const long c = r + i, w = r + cw, e = r + ce;
+    q.un = f.un[c];
+    q.vn = f.vn[c];
+    q.hu = f.hu[c];
+    q.su = f.sshn_u[c];
+    q.hv = f.hv[c];
+    q.sv = f.sshn_v[c];
+    q.e1v = f.dx_v[c];
+    q.e2u = f.dy_u[c];
+    q.ht = f.ht[c];
+    q.st = f.sshn_t[c];
+    q.e1t = f.dx_t[c];
+    q.e2t = f.dy_t[c];
+    q.at = f.area_t[c];
+    q.un_w = f.un[w];
+    q.vn_w = f.vn[w];
+    q.hu_w = f.hu[w];
+    q.su_w = f.sshn_u[w];
+    q.hv_w = f.hv[w];
+    q.sv_w = f.sshn_v[w];
+    q.e1v_w = f.dx_v[w];
+    q.e2u_w = f.dy_u[w];
+    q.un_e = f.un[e];
+    q.vn_e = f.vn[e];
+    q.hu_e = f.hu[e];
+    q.su_e = f.sshn_u[e];
+    q.hv_e = f.hv[e];
+    q.sv_e = f.sshn_v[e];
+    q.e1v_e = f.dx_v[e];
+    q.ht_e = f.ht[e];
+    q.st_e = f.sshn_t[e];
+    q.e1t_e = f.dx_t[e];
+    q.e2t_e = f.dy_t[e];
+    q.at_e = f.area_t[e];
+    q.tm_w = f.tmask[w];
+    q.tm_ee = f.tmask[r + cee];
+    q.tm_n = f.tmask[rn + i];
+    q.tm_ne = f.tmask[rn + ce];
+    const long cp = rp + i;
+    q.dxu = f.dx_u[cp];
+    q.dyv = f.dy_v[cp];
+    q.fcu = f.fcor_u[cp];
+    q.fcv = f.fcor_v[cp];
+    q.aru = f.area_u[cp];
+    q.arv = f.area_v[cp];
+    q.sau = f.ssha_u[cp];
+    q.sav = f.ssha_v[cp];
+}
+
+// PF = prefetch: the loads of row j+2 are issued before the arithmetic of
+// row j, so a lane always has one row of loads in flight while it computes.
+template <int MX, int R, bool PF>
 __global__ __launch_bounds__(MX) void k_fused_march(
     StepIn<double> f, StepScalars<double> p, StepOut<double> o, int W, Bounds b,
     int nstrip, const double *__restrict__ tide_tab, int tide_k, double tide_val)
@@ -52,64 +122,78 @@ __global__ __launch_bounds__(MX) void k_fused_march(
     const int jlo = b.ys - 1 + band * R;
     const int jhi = min(jlo + R, ymax + 1);
 
-    // clamped columns: values of clamped loads are never used (every stencil
-    // of a written point lies inside M x N)
+    // clamped columns and rows: values of clamped loads are never used
+    // (every stencil of a written point lies inside M x N)
     const int cw = max(i - 1, 0), ce = min(i + 1, xmax), cee = min(i + 2, xmax);
+    auto rowoff = [&](int j) { return (long)min(max(j, 0), ymax) * W; };
+    auto fetch = [&](RowRaw<T> &q, int r) {
+        load_raw(q, f, rowoff(r), rowoff(r + 1), rowoff(r - 1), i, cw, ce, cee);
+    };
     const T tide = tide_tab ? tide_tab[tide_k] : tide_val;
 
     // ---- register window ----
     // 3-slot: rows j-1, j, j+1; 2-slot: rows j, j+1; tm: rows j-1 .. j+2
     T un[3], vn[3], hu[3], su[3], hv[3], sv[3], e1v[3], e2u[3];
+    T vn_e[3], hv_e[3], sv_e[3], e1v_e[3];
     T ht[2], st[2], e1t[2], e2t[2], at[2];
     T un_w[2], un_e[2], vn_w[2], hu_w[2], su_w[2], hv_w[2], sv_w[2];
     T e1v_w[2], e2u_w[2], ht_e[2], st_e[2], e1t_e[2], e2t_e[2], at_e[2];
-    T vn_e[3], hv_e[3], sv_e[3], e1v_e[3];
     int tm[4], tm_e[4], tm_w[2];
+    // row j+1 values used once (continuity at i+1) and row-j point values
+    T hu_e1, su_e1, dxu, dyv, fcu, fcv, aru, arv, sau, sav;
+    int tm_ee1;
 
-    auto rowoff = [&](int j) { return (long)min(max(j, 0), ymax) * W; };
-
-    // aligned + shifted values of one row into slot s3 (3-slot) / s2 (2-slot)
-    auto load_row = [&](int j, int s3, int s2) {
-        const long r = rowoff(j);
-        const long c = r + i, w = r + cw, e = r + ce;
-        un[s3] = f.un[c];
-        vn[s3] = f.vn[c];
-        hu[s3] = f.hu[c];
-        su[s3] = f.sshn_u[c];
-        hv[s3] = f.hv[c];
-        sv[s3] = f.sshn_v[c];
-        e1v[s3] = f.dx_v[c];
-        e2u[s3] = f.dy_u[c];
-        ht[s2] = f.ht[c];
-        st[s2] = f.sshn_t[c];
-        e1t[s2] = f.dx_t[c];
-        e2t[s2] = f.dy_t[c];
-        at[s2] = f.area_t[c];
-        un_w[s2] = f.un[w];
-        un_e[s2] = f.un[e];
-        vn_w[s2] = f.vn[w];
-        hu_w[s2] = f.hu[w];
-        su_w[s2] = f.sshn_u[w];
-        hv_w[s2] = f.hv[w];
-        sv_w[s2] = f.sshn_v[w];
-        e1v_w[s2] = f.dx_v[w];
-        e2u_w[s2] = f.dy_u[w];
-        ht_e[s2] = f.ht[e];
-        st_e[s2] = f.sshn_t[e];
-        e1t_e[s2] = f.dx_t[e];
-        e2t_e[s2] = f.dy_t[e];
-        at_e[s2] = f.area_t[e];
-        vn_e[s3] = f.vn[e];
-        hv_e[s3] = f.hv[e];
-        sv_e[s3] = f.sshn_v[e];
-        e1v_e[s3] = f.dx_v[e];
-        tm_w[s2] = f.tmask[w];
+    auto put = [&](const RowRaw<T> &q, int s3, int s2) {
+        un[s3] = q.un;
+        vn[s3] = q.vn;
+        hu[s3] = q.hu;
+        su[s3] = q.su;
+        hv[s3] = q.hv;
+        sv[s3] = q.sv;
+        e1v[s3] = q.e1v;
+        e2u[s3] = q.e2u;
+        vn_e[s3] = q.vn_e;
+        hv_e[s3] = q.hv_e;
+        sv_e[s3] = q.sv_e;
+        e1v_e[s3] = q.e1v_e;
+        if (s2 >= 0) {
+            ht[s2] = q.ht;
+            st[s2] = q.st;
+            e1t[s2] = q.e1t;
+            e2t[s2] = q.e2t;
+            at[s2] = q.at;
+            un_w[s2] = q.un_w;
+            un_e[s2] = q.un_e;
+            vn_w[s2] = q.vn_w;
+            hu_w[s2] = q.hu_w;
+            su_w[s2] = q.su_w;
+            hv_w[s2] = q.hv_w;
+            sv_w[s2] = q.sv_w;
+            e1v_w[s2] = q.e1v_w;
+            e2u_w[s2] = q.e2u_w;
+            ht_e[s2] = q.ht_e;
+            st_e[s2] = q.st_e;
+            e1t_e[s2] = q.e1t_e;
+            e2t_e[s2] = q.e2t_e;
+            at_e[s2] = q.at_e;
+            tm_w[s2] = q.tm_w;
+        }
+        hu_e1 = q.hu_e;
+        su_e1 = q.su_e;
+        tm_ee1 = q.tm_ee;
+        dxu = q.dxu;
+        dyv = q.dyv;
+        fcu = q.fcu;
+        fcv = q.fcv;
+        aru = q.aru;
+        arv = q.arv;
+        sau = q.sau;
+        sav = q.sav;
     };
 
-    // new ssh of T(i, r) and T(i+1, r) from the slots of rows r-1 (s3-1) and
-    // r (s3 / s2), and the tmask rows r-1 .. r+1 (tm slots t-1 .. t+1)
-    auto new_ssh_pair = [&](int r, int s3, int s2, int t, T *sc, T *se) {
-        const long off = rowoff(r);
+    // new ssh of T(i, r) and T(i+1, r) from 3-slot s3 (row r) / s3-1 (row
+    // r-1), 2-slot s2 (row r) and tmask slots t-1 .. t+1 (rows r-1 .. r+1)
+    auto new_ssh_pair = [&](int s3, int s2, int t, T *sc, T *se) {
         ContVals<T> v;
         v.sshn = st[s2];
         v.su_e = su[s3];
@@ -125,13 +209,13 @@ __global__ __launch_bounds__(MX) void k_fused_march(
         v.hv_s = hv[s3 - 1];
         v.vn_s = vn[s3 - 1];
         v.area = at[s2];
-        T x = continuity_val(v, p.rdt);
+        const T x = continuity_val(v, p.rdt);
         *sc = tide_applies(tm[t], tm[t - 1], tm[t + 1], tm_e[t], tm_w[s2]) ? tide : x;
 
         ContVals<T> q;
         q.sshn = st_e[s2];
-        q.su_e = f.sshn_u[off + ce];
-        q.hu_e = f.hu[off + ce];
+        q.su_e = su_e1;
+        q.hu_e = hu_e1;
         q.un_e = un_e[s2];
         q.su_w = su[s3];
         q.hu_w = hu[s3];
@@ -143,43 +227,42 @@ __global__ __launch_bounds__(MX) void k_fused_march(
         q.hv_s = hv_e[s3 - 1];
         q.vn_s = vn_e[s3 - 1];
         q.area = at_e[s2];
-        const int m_ee = f.tmask[off + cee];
-        T y = continuity_val(q, p.rdt);
-        *se = tide_applies(tm_e[t], tm_e[t - 1], tm_e[t + 1], m_ee, tm[t]) ? tide : y;
+        const T y = continuity_val(q, p.rdt);
+        *se = tide_applies(tm_e[t], tm_e[t - 1], tm_e[t + 1], tm_ee1, tm[t]) ? tide : y;
     };
 
-    // ---- prologue: rows jlo-1 (slot 0) and jlo (slot 1 / 0) ----
-    load_row(jlo - 1, 0, 1);  // the 2-slot part of row jlo-1 is overwritten next
-    load_row(jlo, 1, 0);
+    // ---- prologue: rows jlo-1 (3-slot 0) and jlo (3-slot 1, 2-slot 0) ----
+    RowRaw<T> raw;
+    fetch(raw, jlo - 1);
+    put(raw, 0, -1);
+    fetch(raw, jlo);
+    put(raw, 1, 0);
     {
-        const long rm = rowoff(jlo - 1), r0 = rowoff(jlo), r1 = rowoff(jlo + 1);
+        const long rm = rowoff(jlo - 1);
         tm[0] = f.tmask[rm + i];
-        tm[1] = f.tmask[r0 + i];
-        tm[2] = f.tmask[r1 + i];
         tm_e[0] = f.tmask[rm + ce];
+        const long r0 = rowoff(jlo);
+        tm[1] = f.tmask[r0 + i];
         tm_e[1] = f.tmask[r0 + ce];
-        tm_e[2] = f.tmask[r1 + ce];
-        tm_w[0] = f.tmask[r0 + cw];
+        tm[2] = raw.tm_n;
+        tm_e[2] = raw.tm_ne;
     }
-    // new ssh on row jlo: the pair routine reads 3-slot s3=1 (row jlo) and
-    // s3-1=0 (row jlo-1), 2-slot 0 (row jlo), tm slots 0..2
-    T ssh_c, ssh_e;
-    new_ssh_pair(jlo, 1, 0, 1, &ssh_c, &ssh_e);
+    T ssh_c, ssh_e;  // new ssh of T(i, j), T(i+1, j)
+    new_ssh_pair(1, 0, 1, &ssh_c, &ssh_e);
+    if (PF) fetch(raw, jlo + 1);
 
     for (int j = jlo; j < jhi; j++) {
-        // ---- row j+1 into slot 2 / 1, tmask row j+2 into slot 3 ----
-        load_row(j + 1, 2, 1);
-        {
-            const long r2 = rowoff(j + 2);
-            tm[3] = f.tmask[r2 + i];
-            tm_e[3] = f.tmask[r2 + ce];
-        }
-        const long r = rowoff(j);
-        const long c = r + i;
+        // ---- row j+1 into 3-slot 2 / 2-slot 1, tmask row j+2 into slot 3 ----
+        if (!PF) fetch(raw, j + 1);
+        put(raw, 2, 1);
+        tm[3] = raw.tm_n;
+        tm_e[3] = raw.tm_ne;
+        if (PF) fetch(raw, j + 2);  // in flight during the arithmetic below
+        const long c = rowoff(j) + i;
 
         // ---- new ssh of T(i, j+1), T(i+1, j+1) ----
         T ssh_n, ssh_ne;
-        new_ssh_pair(j + 1, 2, 1, 2, &ssh_n, &ssh_ne);
+        new_ssh_pair(2, 1, 2, &ssh_n, &ssh_ne);
 
         // ---- T(i,j) ----
         if (in_rect(i, j, b.xs, b.xe, b.ys, b.ye)) o.ssh[c] = ssh_c;
@@ -234,10 +317,10 @@ __global__ __launch_bounds__(MX) void k_fused_march(
                 v.e2u_c = e2u[1];
                 v.e2u_s = e2u[0];
                 v.e2u_n = e2u[2];
-                v.e1u_c = f.dx_u[c];
-                v.fcor = f.fcor_u[c];
-                v.area = f.area_u[c];
-                v.ssha = f.ssha_u[c];
+                v.e1u_c = dxu;
+                v.fcor = fcu;
+                v.area = aru;
+                v.ssha = sau;
                 wrote = momentum_u_val(v, p, &val);
             }
             if (!wrote && in_rect(i, j, b.xs - 1, b.xe, b.ys - 1, b.ye + 1) &&
@@ -298,10 +381,10 @@ __global__ __launch_bounds__(MX) void k_fused_march(
                 v.sv_w = sv_w[0];
                 v.hv_e = hv_e[1];
                 v.sv_e = sv_e[1];
-                v.e2v_c = f.dy_v[c];
-                v.fcor = f.fcor_v[c];
-                v.area = f.area_v[c];
-                v.ssha = f.ssha_v[c];
+                v.e2v_c = dyv;
+                v.fcor = fcv;
+                v.area = arv;
+                v.ssha = sav;
                 wrote = momentum_v_val(v, p, &val);
             }
             if (!wrote && in_rect(i, j, b.xs - 1, b.xe + 1, b.ys - 1, b.ye) &&
@@ -315,14 +398,12 @@ __global__ __launch_bounds__(MX) void k_fused_march(
         // ---- next_sshu(i,j), next_sshv(i,j) from the new ssh ----
         if (in_rect(i, j, b.xs, b.xe - 1, b.ys, b.ye)) {
             T val;
-            if (next_ssh_pt(ssh_c, ssh_e, at[0], at_e[0], f.area_u[c], tm[1],
-                            tm_e[1], &val))
+            if (next_ssh_pt(ssh_c, ssh_e, at[0], at_e[0], aru, tm[1], tm_e[1], &val))
                 o.sshu[c] = val;
         }
         if (in_rect(i, j, b.xs, b.xe, b.ys, b.ye - 1)) {
             T val;
-            if (next_ssh_pt(ssh_c, ssh_n, at[0], at[1], f.area_v[c], tm[1], tm[2],
-                            &val))
+            if (next_ssh_pt(ssh_c, ssh_n, at[0], at[1], arv, tm[1], tm[2], &val))
                 o.sshv[c] = val;
         }
 
@@ -352,11 +433,291 @@ __global__ __launch_bounds__(MX) void k_fused_march(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Overlapped-wavefront march. One block = one wavefront of 64 lanes over 64
+// consecutive columns, of which lanes 1..61 own their column's outputs and
+// lanes 0, 62, 63 are halo (neighbouring strips overlap by 3 columns). Every
+// value a lane needs at column i-1 or i+1 sits in lane l-1 / l+1 of the same
+// wavefront, so it is taken with a cross-lane permute (ds_bpermute) rather
+// than loaded or kept in a second register: each lane carries only its own
+// column's window, which roughly halves the registers of the other march and
+// raises the wavefronts per SIMD. The new ssh of T(i+1) comes from lane l+1,
+// which computes it for its own column (lane 62 is the last one that needs
+// it, and its stencil ends at lane 63).
+constexpr int WAVE_OUT = 61;
+
+__device__ __forceinline__ double from_below(double x) { return __shfl_up(x, 1, 64); }
+__device__ __forceinline__ double from_above(double x) { return __shfl_down(x, 1, 64); }
+__device__ __forceinline__ int from_below(int x) { return __shfl_up(x, 1, 64); }
+__device__ __forceinline__ int from_above(int x) { return __shfl_down(x, 1, 64); }
+
+template <int R, int MINW>
+__global__ __launch_bounds__(64, MINW) void k_fused_wave(
+    StepIn<double> f, StepScalars<double> p, StepOut<double> o, int W, Bounds b,
+    int nstrip, const double *__restrict__ tide_tab, int tide_k, double tide_val)
+{
+    using T = double;
+    const int nblk = (int)gridDim.x, bid = (int)blockIdx.x;
+    const int xcd = bid & 7, k = bid >> 3;
+    const int per = nblk >> 3, extra = nblk & 7;
+    const int logical = xcd * per + min(xcd, extra) + k;
+    const int band = logical / nstrip, strip = logical - band * nstrip;
+
+    const int xmax = b.xe + 1, ymax = b.ye + 1;
+    const int lane = (int)threadIdx.x;
+    const int i = b.xs - 1 + strip * WAVE_OUT + lane - 1;  // may be -1 or > xmax
+    const bool owner = lane >= 1 && lane <= WAVE_OUT && i <= xmax;
+    const int ci = min(max(i, 0), xmax);                 // clamped load column
+    const int jlo = b.ys - 1 + band * R;
+    const int jhi = min(jlo + R, ymax + 1);
+    auto rowoff = [&](int j) { return (long)min(max(j, 0), ymax) * W; };
+    const T tide = tide_tab ? tide_tab[tide_k] : tide_val;
+
+    // own-column window: 3-slot rows j-1, j, j+1; 2-slot rows j, j+1;
+    // tm rows j-1 .. j+2
+    T un[3], vn[3], hu[3], su[3], hv[3], sv[3], e1v[3], e2u[3];
+    T ht[2], st[2], e1t[2], e2t[2], at[2];
+    int tm[4];
+
+    auto load3 = [&](int j, int s) {
+        const long c = rowoff(j) + ci;
+        un[s] = f.un[c];
+        vn[s] = f.vn[c];
+        hu[s] = f.hu[c];
+        su[s] = f.sshn_u[c];
+        hv[s] = f.hv[c];
+        sv[s] = f.sshn_v[c];
+        e1v[s] = f.dx_v[c];
+        e2u[s] = f.dy_u[c];
+    };
+    auto load2 = [&](int j, int s) {
+        const long c = rowoff(j) + ci;
+        ht[s] = f.ht[c];
+        st[s] = f.sshn_t[c];
+        e1t[s] = f.dx_t[c];
+        e2t[s] = f.dy_t[c];
+        at[s] = f.area_t[c];
+    };
+    // new ssh of T(i, r): 3-slot s (row r) and s-1 (row r-1), 2-slot s2,
+    // tm slots t-1..t+1; every lane evaluates it for its own column
+    auto new_ssh = [&](int s, int s2, int t) {
+        ContVals<T> v;
+        v.sshn = st[s2];
+        v.su_e = su[s];
+        v.hu_e = hu[s];
+        v.un_e = un[s];
+        v.su_w = from_below(su[s]);
+        v.hu_w = from_below(hu[s]);
+        v.un_w = from_below(un[s]);
+        v.sv_n = sv[s];
+        v.hv_n = hv[s];
+        v.vn_n = vn[s];
+        v.sv_s = sv[s - 1];
+        v.hv_s = hv[s - 1];
+        v.vn_s = vn[s - 1];
+        v.area = at[s2];
+        const int m_w = from_below(tm[t]), m_e = from_above(tm[t]);
+        const T x = continuity_val(v, p.rdt);
+        return tide_applies(tm[t], tm[t - 1], tm[t + 1], m_e, m_w) ? tide : x;
+    };
+
+    // ---- prologue ----
+    load3(jlo - 1, 0);
+    load3(jlo, 1);
+    load2(jlo, 0);
+    tm[0] = f.tmask[rowoff(jlo - 1) + ci];
+    tm[1] = f.tmask[rowoff(jlo) + ci];
+    tm[2] = f.tmask[rowoff(jlo + 1) + ci];
+    T ssh_c = new_ssh(1, 0, 1);  // T(i, jlo)
+
+    for (int j = jlo; j < jhi; j++) {
+        load3(j + 1, 2);
+        load2(j + 1, 1);
+        tm[3] = f.tmask[rowoff(j + 2) + ci];
+        const long c = rowoff(j) + ci;
+        const T dxu = f.dx_u[c], dyv = f.dy_v[c], fcu = f.fcor_u[c], fcv = f.fcor_v[c];
+        const T aru = f.area_u[c], arv = f.area_v[c], sau = f.ssha_u[c], sav = f.ssha_v[c];
+
+        const T ssh_n = new_ssh(2, 1, 2);   // T(i, j+1)
+        const T ssh_e = from_above(ssh_c);  // T(i+1, j)
+
+        // Every lane runs every phase (the permutes are wavefront-wide);
+        // only the stores depend on the lane owning its column. Each phase
+        // takes its neighbour values right before use, which keeps them
+        // short-lived.
+        const bool own_t = owner && in_rect(i, j, b.xs, b.xe, b.ys, b.ye);
+        if (own_t) o.ssh[c] = ssh_c;
+
+        const int tm_e0 = from_above(tm[1]);
+        const int tm_e1 = from_above(tm[2]);
+        {  // U(i,j): momentum_u, else bc_solid_u
+            MomUVals<T> v;
+            v.m_c = tm[1];
+            v.m_e = tm_e0;
+            v.m_s = tm[0];
+            v.m_se = from_above(tm[0]);
+            v.m_n = tm[2];
+            v.m_ne = tm_e1;
+            v.un_c = un[1];
+            v.un_e = from_above(un[1]);
+            v.un_w = from_below(un[1]);
+            v.un_s = un[0];
+            v.un_n = un[2];
+            v.vn_s = vn[0];
+            v.vn_se = from_above(vn[0]);
+            v.vn_c = vn[1];
+            v.vn_e = from_above(vn[1]);
+            v.e1v_s = e1v[0];
+            v.e1v_se = from_above(e1v[0]);
+            v.e1v_c = e1v[1];
+            v.e1v_e = from_above(e1v[1]);
+            v.hv_s = hv[0];
+            v.sv_s = sv[0];
+            v.hv_se = from_above(hv[0]);
+            v.sv_se = from_above(sv[0]);
+            v.hv_c = hv[1];
+            v.sv_c = sv[1];
+            v.hv_e = from_above(hv[1]);
+            v.sv_e = from_above(sv[1]);
+            v.e2t_c = e2t[0];
+            v.e2t_e = from_above(e2t[0]);
+            v.e1t_c = e1t[0];
+            v.e1t_e = from_above(e1t[0]);
+            v.ht_c = ht[0];
+            v.ht_e = from_above(ht[0]);
+            v.st_c = st[0];
+            v.st_e = from_above(st[0]);
+            v.hu_c = hu[1];
+            v.su_c = su[1];
+            v.hu_s = hu[0];
+            v.su_s = su[0];
+            v.hu_n = hu[2];
+            v.su_n = su[2];
+            v.e2u_c = e2u[1];
+            v.e2u_s = e2u[0];
+            v.e2u_n = e2u[2];
+            v.e1u_c = dxu;
+            v.fcor = fcu;
+            v.area = aru;
+            v.ssha = sau;
+            T val = T(0);
+            bool wrote = in_rect(i, j, b.xs, b.xe - 1, b.ys, b.ye) &&
+                         momentum_u_val(v, p, &val);
+            if (!wrote && in_rect(i, j, b.xs - 1, b.xe, b.ys - 1, b.ye + 1) &&
+                tm[1] * tm_e0 == 0) {
+                val = T(0);
+                wrote = true;
+            }
+            if (owner && wrote) o.u[c] = val;
+        }
+        {  // V(i,j): momentum_v, else bc_solid_v
+            MomVVals<T> v;
+            v.m_c = tm[1];
+            v.m_e = tm_e0;
+            v.m_n = tm[2];
+            v.m_w = from_below(tm[1]);
+            v.m_nw = from_below(tm[2]);
+            v.m_ne = tm_e1;
+            v.vn_c = vn[1];
+            v.vn_n = vn[2];
+            v.vn_s = vn[0];
+            v.vn_w = from_below(vn[1]);
+            v.vn_e = from_above(vn[1]);
+            v.un_w = from_below(un[1]);
+            v.un_nw = from_below(un[2]);
+            v.un_c = un[1];
+            v.un_n = un[2];
+            v.e2u_w = from_below(e2u[1]);
+            v.e2u_nw = from_below(e2u[2]);
+            v.e2u_c = e2u[1];
+            v.e2u_n = e2u[2];
+            v.hu_w = from_below(hu[1]);
+            v.su_w = from_below(su[1]);
+            v.hu_nw = from_below(hu[2]);
+            v.su_nw = from_below(su[2]);
+            v.hu_c = hu[1];
+            v.su_c = su[1];
+            v.hu_n = hu[2];
+            v.su_n = su[2];
+            v.e1t_c = e1t[0];
+            v.e1t_n = e1t[1];
+            v.e2t_c = e2t[0];
+            v.e2t_n = e2t[1];
+            v.ht_c = ht[0];
+            v.ht_n = ht[1];
+            v.st_c = st[0];
+            v.st_n = st[1];
+            v.e1v_c = e1v[1];
+            v.e1v_w = from_below(e1v[1]);
+            v.e1v_e = from_above(e1v[1]);
+            v.hv_c = hv[1];
+            v.sv_c = sv[1];
+            v.hv_w = from_below(hv[1]);
+            v.sv_w = from_below(sv[1]);
+            v.hv_e = from_above(hv[1]);
+            v.sv_e = from_above(sv[1]);
+            v.e2v_c = dyv;
+            v.fcor = fcv;
+            v.area = arv;
+            v.ssha = sav;
+            T val = T(0);
+            bool wrote = in_rect(i, j, b.xs, b.xe, b.ys, b.ye - 1) &&
+                         momentum_v_val(v, p, &val);
+            if (!wrote && in_rect(i, j, b.xs - 1, b.xe + 1, b.ys - 1, b.ye) &&
+                tm[1] * tm[2] == 0) {
+                val = T(0);
+                wrote = true;
+            }
+            if (owner && wrote) o.v[c] = val;
+        }
+        {  // next_sshu(i,j), next_sshv(i,j)
+            const T at_e0 = from_above(at[0]);
+            T val;
+            if (owner && in_rect(i, j, b.xs, b.xe - 1, b.ys, b.ye) &&
+                next_ssh_pt(ssh_c, ssh_e, at[0], at_e0, aru, tm[1], tm_e0, &val))
+                o.sshu[c] = val;
+            if (owner && in_rect(i, j, b.xs, b.xe, b.ys, b.ye - 1) &&
+                next_ssh_pt(ssh_c, ssh_n, at[0], at[1], arv, tm[1], tm[2], &val))
+                o.sshv[c] = val;
+        }
+
+#define NEMO_SHIFT3(a) \
+    a[0] = a[1];       \
+    a[1] = a[2];
+#define NEMO_SHIFT2(a) a[0] = a[1];
+        NEMO_SHIFT3(un) NEMO_SHIFT3(vn) NEMO_SHIFT3(hu) NEMO_SHIFT3(su)
+        NEMO_SHIFT3(hv) NEMO_SHIFT3(sv) NEMO_SHIFT3(e1v) NEMO_SHIFT3(e2u)
+        NEMO_SHIFT2(ht) NEMO_SHIFT2(st) NEMO_SHIFT2(e1t) NEMO_SHIFT2(e2t) NEMO_SHIFT2(at)
+#undef NEMO_SHIFT3
+#undef NEMO_SHIFT2
+        tm[0] = tm[1];
+        tm[1] = tm[2];
+        tm[2] = tm[3];
+        ssh_c = ssh_n;
+    }
+}
+
+template <int R, int MINW>
+void launch_wave_t(hipStream_t s, const StepIn<double> &f,
+                   const StepScalars<double> &p, const StepOut<double> &o, int W,
+                   Bounds b, const double *tide_tab, int tide_k, double tide_val)
+{
+    const int ncol = b.xe - b.xs + 3, nrow = b.ye - b.ys + 3;
+    const int nstrip = (ncol + WAVE_OUT - 1) / WAVE_OUT, nband = (nrow + R - 1) / R;
+    k_fused_wave<R, MINW><<<dim3((unsigned)(nstrip * nband)), dim3(64), 0, s>>>(
+        f, p, o, W, b, nstrip, tide_tab, tide_k, tide_val);
+    NEMO_LAUNCH_CHECK();
+}
+
+// kind: 0 register-window march, 1 the same with the row loads issued one
+// row ahead, 2 overlapped-wavefront march (61 output columns per block),
+// 3 / 4 the overlapped-wavefront march compiled for >= 3 / 4 wavefronts per
+// SIMD (the compiler keeps its registers within that budget)
 struct MarchShape {
-    int mx, rows;
+    int mx, rows, prefetch;
 };
 
-template <int MX, int R>
+template <int MX, int R, bool PF>
 void launch_march_t(hipStream_t s, const StepIn<double> &f,
                     const StepScalars<double> &p, const StepOut<double> &o,
                     int W, Bounds b, const double *tide_tab, int tide_k,
@@ -364,26 +725,31 @@ void launch_march_t(hipStream_t s, const StepIn<double> &f,
 {
     const int ncol = b.xe - b.xs + 3, nrow = b.ye - b.ys + 3;
     const int nstrip = (ncol + MX - 1) / MX, nband = (nrow + R - 1) / R;
-    k_fused_march<MX, R><<<dim3((unsigned)(nstrip * nband)), dim3(MX), 0, s>>>(
+    k_fused_march<MX, R, PF><<<dim3((unsigned)(nstrip * nband)), dim3(MX), 0, s>>>(
         f, p, o, W, b, nstrip, tide_tab, tide_k, tide_val);
     NEMO_LAUNCH_CHECK();
 }
 
 }  // namespace
 
-// Variants: index -> (block width MX, rows per band R).
+// Variants: index -> (block width MX, rows per band R, prefetch).
 static const MarchShape kShapes[] = {
-    {256, 32}, {256, 64}, {256, 16}, {128, 32}, {128, 64}, {64, 64}, {64, 128},
+    {256, 32, 0}, {256, 64, 0}, {256, 16, 0}, {128, 32, 0}, {128, 64, 0},
+    {64, 64, 0},  {64, 128, 0}, {256, 32, 1}, {256, 64, 1}, {256, 16, 1},
+    {128, 32, 1}, {64, 64, 1},  {64, 32, 1},  {64, 16, 2},  {64, 32, 2},
+    {64, 64, 2},  {64, 128, 2}, {64, 32, 3},  {64, 64, 3},  {64, 32, 4},
+    {64, 64, 4},
 };
 
 int march_variant_count() { return (int)(sizeof(kShapes) / sizeof(kShapes[0])); }
 
-void march_variant_shape(int variant, int *mx, int *rows)
+void march_variant_shape(int variant, int *mx, int *rows, int *prefetch)
 {
     if (variant < 0 || variant >= march_variant_count())
         NEMO_FATAL("unknown fused march variant %d", variant);
     *mx = kShapes[variant].mx;
     *rows = kShapes[variant].rows;
+    *prefetch = kShapes[variant].prefetch;
 }
 
 void launch_fused_march(hipStream_t s, int variant, const StepIn<double> &f,
@@ -391,16 +757,35 @@ void launch_fused_march(hipStream_t s, int variant, const StepIn<double> &f,
                         int W, Bounds b, const double *tide_tab, int tide_k,
                         double tide_val)
 {
+#define NEMO_MARCH_CASE(n, MX, R, PF)                                           \
+    case n:                                                                    \
+        launch_march_t<MX, R, PF>(s, f, p, o, W, b, tide_tab, tide_k, tide_val); \
+        break;
     switch (variant) {
-    case 0: launch_march_t<256, 32>(s, f, p, o, W, b, tide_tab, tide_k, tide_val); break;
-    case 1: launch_march_t<256, 64>(s, f, p, o, W, b, tide_tab, tide_k, tide_val); break;
-    case 2: launch_march_t<256, 16>(s, f, p, o, W, b, tide_tab, tide_k, tide_val); break;
-    case 3: launch_march_t<128, 32>(s, f, p, o, W, b, tide_tab, tide_k, tide_val); break;
-    case 4: launch_march_t<128, 64>(s, f, p, o, W, b, tide_tab, tide_k, tide_val); break;
-    case 5: launch_march_t<64, 64>(s, f, p, o, W, b, tide_tab, tide_k, tide_val); break;
-    case 6: launch_march_t<64, 128>(s, f, p, o, W, b, tide_tab, tide_k, tide_val); break;
+        NEMO_MARCH_CASE(0, 256, 32, false)
+        NEMO_MARCH_CASE(1, 256, 64, false)
+        NEMO_MARCH_CASE(2, 256, 16, false)
+        NEMO_MARCH_CASE(3, 128, 32, false)
+        NEMO_MARCH_CASE(4, 128, 64, false)
+        NEMO_MARCH_CASE(5, 64, 64, false)
+        NEMO_MARCH_CASE(6, 64, 128, false)
+        NEMO_MARCH_CASE(7, 256, 32, true)
+        NEMO_MARCH_CASE(8, 256, 64, true)
+        NEMO_MARCH_CASE(9, 256, 16, true)
+        NEMO_MARCH_CASE(10, 128, 32, true)
+        NEMO_MARCH_CASE(11, 64, 64, true)
+        NEMO_MARCH_CASE(12, 64, 32, true)
+    case 13: launch_wave_t<16, 1>(s, f, p, o, W, b, tide_tab, tide_k, tide_val); break;
+    case 14: launch_wave_t<32, 1>(s, f, p, o, W, b, tide_tab, tide_k, tide_val); break;
+    case 15: launch_wave_t<64, 1>(s, f, p, o, W, b, tide_tab, tide_k, tide_val); break;
+    case 16: launch_wave_t<128, 1>(s, f, p, o, W, b, tide_tab, tide_k, tide_val); break;
+    case 17: launch_wave_t<32, 3>(s, f, p, o, W, b, tide_tab, tide_k, tide_val); break;
+    case 18: launch_wave_t<64, 3>(s, f, p, o, W, b, tide_tab, tide_k, tide_val); break;
+    case 19: launch_wave_t<32, 4>(s, f, p, o, W, b, tide_tab, tide_k, tide_val); break;
+    case 20: launch_wave_t<64, 4>(s, f, p, o, W, b, tide_tab, tide_k, tide_val); break;
     default: NEMO_FATAL("unknown fused march variant %d", variant);
     }
+#undef NEMO_MARCH_CASE
 }
 
 }  // namespace nemo

@@ -634,14 +634,16 @@ void nemo_hip_set_fused_variant(int variant)
 
 int nemo_hip_fused_variant(void) { return g_fused_variant; }
 
-void nemo_hip_fused_variant_shape(int variant, int *block_width, int *rows)
+void nemo_hip_fused_variant_shape(int variant, int *block_width, int *rows,
+                                  int *prefetch)
 {
     if (variant == -1) {
         *block_width = 64;
         *rows = 4;
+        *prefetch = 0;
         return;
     }
-    march_variant_shape(variant, block_width, rows);
+    march_variant_shape(variant, block_width, rows, prefetch);
 }
 
 double nemo_hip_fused_kernel_ms(void) { return g_fused_ms; }

@@ -84,7 +84,7 @@ void launch_fused_march(hipStream_t s, int variant, const StepIn<double> &f,
                         int W, Bounds b, const double *tide_tab, int tide_k,
                         double tide_val);
 int march_variant_count();
-void march_variant_shape(int variant, int *mx, int *rows);
+void march_variant_shape(int variant, int *mx, int *rows, int *prefetch);
 
 // Small helpers.
 void launch_fill_int(hipStream_t s, int *p, int n, int value);
