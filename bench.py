@@ -35,6 +35,7 @@ METRIC = ("grid-pt updates/s + % HBM peak at 4096²; "
           "FP32/learned vs plain FP64, same err")
 HBM_PEAK_GBS = 8000.0
 B_ALG_F64 = 212          # bytes per interior grid-point update, FP64
+B_ALG_MIXED = 116        # FP32 fields + the ssh compensation word
 PROFILE_TRAFFIC = ROOT / "profiles" / "fused_step_f64_4096_traffic.txt"
 
 
@@ -51,6 +52,7 @@ def parse_args():
     p.add_argument("--cpu-seconds", type=float, default=12.0,
                    help="target CPU time of the cpu_baseline sample")
     p.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
+    p.add_argument("--no-mixed", action="store_true", help="skip the FP32 mixed leg")
     return p.parse_args()
 
 
@@ -103,6 +105,75 @@ def cpu_baseline(n: int, seconds: float) -> dict:
     }
 
 
+def rel_err(x, ref):
+    import numpy as np
+    d = x - ref
+    return (float(np.max(np.abs(d)) / max(np.max(np.abs(ref)), 1e-300)),
+            float(np.linalg.norm(d) / max(np.linalg.norm(ref), 1e-300)))
+
+
+def mixed_leg(pristine, a, pts, f64_ms_per_step, f64_kernel_ms, f64_variant) -> dict:
+    """The FP32 mixed form on the same workload: throughput (same timing
+    protocol as the FP64 line), kernel roofline against its 116 B per
+    update, and its error against the FP64 form (bitwise the oracle) after
+    250 / 500 / 1000 steps."""
+    from psyclonebench_amd import _native
+    from psyclonebench_amd.psy import MixedStepper, TimeStepper
+
+    lib = _native.hip_lib()
+    st = MixedStepper(pristine.copy())
+    st.run(1 + a.warmup)
+    tune = {}
+    for rows in (16, 32, 64, 128):
+        lib.nemo_mp_set_rows(st.handle, rows)
+        tune[rows] = st.timed_run(a.tune_steps)
+    rows = min(tune, key=tune.get)
+    lib.nemo_mp_set_rows(st.handle, rows)
+    st.run(a.steps)
+    lib.nemo_hip_synchronize()
+    t0 = time.perf_counter()
+    st.run(a.steps)
+    lib.nemo_hip_synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = st.timed_run(a.kernel_steps)
+    st.release()
+
+    # accuracy against FP64
+    g = pristine.grid
+    sl = (slice(g.ys, g.ye + 1), slice(g.xs, g.xe + 1))
+    lib.nemo_hip_set_fused_variant(f64_variant)
+    ref = TimeStepper(pristine.copy(), mode=2)
+    mp = MixedStepper(pristine.copy(), rows=rows)
+    errs, done = {}, 0
+    for cp in (250, 500, 1000):
+        ref.run(cp - done)
+        mp.run(cp - done)
+        done = cp
+        errs[str(cp)] = {n: rel_err(mp.fetch(n)[sl], ref.fetch(n)[sl])
+                         for n in ("sshn_t", "un", "vn")}
+    ref.release()
+    mp.release()
+    b_alg = B_ALG_MIXED * pts
+    achieved = b_alg / (kernel_ms * 1e-3) / 1e9
+    ms_step = wall * 1e3 / a.steps
+    return {
+        "value": pts * a.steps / wall,
+        "unit": "grid-pt updates/s",
+        "dtype": "f32 fields, compensated f32 ssh with f64 accumulation, f64 tide",
+        "ms_per_step": ms_step,
+        "speedup_vs_f64": f64_ms_per_step / ms_step,
+        "kernel_speedup_vs_f64": f64_kernel_ms / kernel_ms,
+        "rows_per_band": rows,
+        "rows_kernel_ms": {str(k): round(v, 4) for k, v in tune.items()},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "kernel": "k_mixed_wave", "kernel_ms": kernel_ms,
+                     "bytes_per_launch_alg": b_alg},
+        "error_vs_f64": {"metric": "(relmax, relrms) over the internal T region",
+                         "after_steps": errs},
+    }
+
+
 def main():
     a = parse_args()
     if a.gpus != 1:
@@ -121,7 +192,8 @@ def main():
         raise SystemExit(f"bench.py: no GPU visible ({info})")
 
     cfg = c2_config(a.n, a.warmup + a.steps, seed=0)
-    model = Model(cfg)
+    pristine = Model(cfg)          # initial state, copied for every leg
+    model = pristine.copy()
     g = model.grid
     pts = (g.M - 2) * (g.N - 2)
     st = TimeStepper(model, mode=2)
@@ -197,6 +269,8 @@ def main():
             "bytes_per_launch_alg": bytes_alg,
         },
     }
+    if not a.no_mixed:
+        out["mixed"] = mixed_leg(pristine, a, pts, wall * 1e3 / a.steps, kernel_ms, best)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(a.n, a.cpu_seconds)
     print(json.dumps(out), flush=True)

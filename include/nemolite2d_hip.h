@@ -153,6 +153,31 @@ NEMO_API int nemo_hip_fused_variant(void);
 NEMO_API void nemo_hip_fused_variant_shape(int variant, int *block_width,
                                            int *rows, int *prefetch);
 
+/* ---- mixed-precision form (no reference counterpart) -------------------
+ * FP32 storage and arithmetic for velocities, face ssh, depths and metrics;
+ * the T-cell ssh is a compensated pair of FP32 words updated with FP64
+ * accumulation; the tidal elevation amp*sin(omega_tide*istep*rdt) is FP64.
+ * Takes the same 25 FP64 host arrays as c_invoke_time_step (converted on
+ * upload) and returns an opaque handle. Requires the same mask condition as
+ * the fused form (exits otherwise). */
+NEMO_API void *nemo_mp_create(NEMO_HOST_ARRAYS, int internal_xstart,
+                              int internal_xstop, int internal_ystart,
+                              int internal_ystop, int width, int total_size,
+                              double rdt, double cbfr, double visc,
+                              double omega, double d2r, double g,
+                              double tide_amp, double tide_omega);
+/* nsteps steps istep, istep+1, ... as one hipGraph replay (asynchronous). */
+NEMO_API void nemo_mp_steps(void *handle, int nsteps, int istep);
+/* The same steps as plain launches, each timed with HIP events; returns the
+ * mean duration in ms of the mixed step kernel. */
+NEMO_API double nemo_mp_kernel_ms(void *handle, int nsteps, int istep);
+/* Rows per band of the kernel: 16, 32 (default), 64 or 128. */
+NEMO_API void nemo_mp_set_rows(void *handle, int rows);
+/* Blocking read-back as FP64 (total_size values): field 0 sshn_t (hi+lo),
+ * 1 un, 2 vn, 3 sshn_u, 4 sshn_v. */
+NEMO_API void nemo_mp_read(void *handle, int field, double *out);
+NEMO_API void nemo_mp_destroy(void *handle);
+
 /* Releases the device buffers, graphs and lists of one context (identified
  * by its ssha_t slot), or of all contexts. Slots become dangling. */
 NEMO_API void nemo_hip_release(void **ssha_t_device);

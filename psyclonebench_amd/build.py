@@ -37,7 +37,8 @@ HIPCC_FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 
-HIP_SOURCES = ["nemo_step.hip", "nemo_march.hip", "nemo_psy.cpp"]
+HIP_SOURCES = ["nemo_step.hip", "nemo_march.hip", "nemo_psy.cpp", "nemo_mixed.hip",
+               "nemo_mixed_api.cpp"]
 
 
 def _hipcc() -> str:

@@ -1,0 +1,160 @@
+// Host-side helpers shared by the FP64 PSy layer (nemo_psy.cpp) and the
+// mixed-precision and ensemble APIs: argument order of the reference C ABI,
+// geometry checks, the library stream, device allocation and the Flather
+// segment lists.
+#pragma once
+
+#include <cmath>
+#include <vector>
+
+#include "nemo_hip_common.h"
+#include "nemo_step.h"
+
+namespace nemo {
+
+// Argument order of the reference C ABI (psykal_cpp/time_step.cpp:12-37).
+enum Arr {
+    A_SSHA_T, A_SSHN_T, A_SSHN_U, A_SSHN_V, A_HU, A_HV, A_UN, A_VN, A_UA,
+    A_HT, A_SSHA_U, A_VA, A_SSHA_V, A_TMASK, A_AREA_T, A_AREA_U, A_AREA_V,
+    A_DX_U, A_DX_V, A_DX_T, A_DY_U, A_DY_V, A_DY_T, A_GPHIU, A_GPHIV,
+    A_COUNT
+};
+
+struct Geometry {
+    int xs, xe, ys, ye, W, total;
+};
+
+struct Physics {
+    double rdt, cbfr, visc, omega, d2r, g;
+};
+
+// The library's one stream (default priority, non-blocking w.r.t. stream 0).
+hipStream_t lib_stream();
+
+template <typename T>
+inline T *dev_alloc(size_t n)
+{
+    void *p = nullptr;
+    NEMO_HIP_CHECK(hipMalloc(&p, n * sizeof(T)));
+    return static_cast<T *>(p);
+}
+
+struct HostFlather {
+    std::vector<int> seg_ptr{0}, idx, off;
+    std::vector<double> coef;
+};
+
+// Faces where bc_flather_u (stride 1) or bc_flather_v (stride W) writes,
+// grouped per row (U) or per column (V) in the reference's sweep order.
+// boundary_conditions_kern.c:282-291 / :322-331.
+inline HostFlather build_flather(const int *tm, const double *h, const Geometry &G,
+                          double g, bool along_i)
+{
+    HostFlather L;
+    const int W = G.W;
+    const int stride = along_i ? 1 : W;
+    // U faces: ji in [xs-1, xe], jj in [ys-1, ye+1];
+    // V faces: ji in [xs-1, xe+1], jj in [ys-1, ye]
+    const int x0 = G.xs - 1, y0 = G.ys - 1;
+    const int x1 = along_i ? G.xe : G.xe + 1;
+    const int y1 = along_i ? G.ye + 1 : G.ye;
+    const int outer0 = along_i ? y0 : x0, outer1 = along_i ? y1 : x1;
+    const int inner0 = along_i ? x0 : y0, inner1 = along_i ? x1 : y1;
+    for (int o = outer0; o <= outer1; o++) {
+        const size_t before = L.idx.size();
+        for (int in = inner0; in <= inner1; in++) {
+            const int ji = along_i ? in : o, jj = along_i ? o : in;
+            const long c = (long)jj * W + ji;
+            const int mc = tm[c], mp = tm[c + stride];
+            if (mc + mp <= -1) continue;
+            int d;
+            if (mc < 0)
+                d = stride;
+            else if (mp < 0)
+                d = -stride;
+            else
+                continue;
+            const long q = c + d;
+            L.idx.push_back((int)c);
+            L.off.push_back(q >= 0 && q < G.total ? d : 0);
+            L.coef.push_back(std::sqrt(g / h[c]));
+        }
+        if (L.idx.size() > before) L.seg_ptr.push_back((int)L.idx.size());
+    }
+    return L;
+}
+
+struct DeviceFlather {
+    int *seg_ptr = nullptr, *idx = nullptr, *off = nullptr;
+    double *coef = nullptr;
+    int nseg = 0;
+
+    void upload(const HostFlather &L, hipStream_t s)
+    {
+        release();
+        nseg = (int)L.seg_ptr.size() - 1;
+        if (nseg == 0) return;
+        seg_ptr = dev_alloc<int>(L.seg_ptr.size());
+        idx = dev_alloc<int>(L.idx.size());
+        off = dev_alloc<int>(L.off.size());
+        coef = dev_alloc<double>(L.coef.size());
+        NEMO_HIP_CHECK(hipMemcpyAsync(seg_ptr, L.seg_ptr.data(),
+                                      L.seg_ptr.size() * sizeof(int),
+                                      hipMemcpyHostToDevice, s));
+        NEMO_HIP_CHECK(hipMemcpyAsync(idx, L.idx.data(), L.idx.size() * sizeof(int),
+                                      hipMemcpyHostToDevice, s));
+        NEMO_HIP_CHECK(hipMemcpyAsync(off, L.off.data(), L.off.size() * sizeof(int),
+                                      hipMemcpyHostToDevice, s));
+        NEMO_HIP_CHECK(hipMemcpyAsync(coef, L.coef.data(),
+                                      L.coef.size() * sizeof(double),
+                                      hipMemcpyHostToDevice, s));
+        NEMO_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    void release()
+    {
+        if (seg_ptr) NEMO_HIP_CHECK(hipFree(seg_ptr));
+        if (idx) NEMO_HIP_CHECK(hipFree(idx));
+        if (off) NEMO_HIP_CHECK(hipFree(off));
+        if (coef) NEMO_HIP_CHECK(hipFree(coef));
+        seg_ptr = idx = off = nullptr;
+        coef = nullptr;
+        nseg = 0;
+    }
+    FlatherList view() const { return FlatherList{seg_ptr, idx, off, coef, nseg}; }
+};
+
+inline void check_geometry(const Geometry &G)
+{
+    if (G.W <= 0 || G.xs < 1 || G.ys < 1 || G.xe < G.xs || G.ye < G.ys)
+        NEMO_FATAL("invalid internal bounds %d..%d x %d..%d (width %d)", G.xs,
+                   G.xe, G.ys, G.ye, G.W);
+    if (G.xe + 2 > G.W)
+        NEMO_FATAL("width %d too small for internal xstop %d", G.W, G.xe);
+    if ((long)G.W * (G.ye + 2) > (long)G.total)
+        NEMO_FATAL("total_size %d too small for width %d and ystop %d", G.total,
+                   G.W, G.ye);
+}
+
+// Coriolis factor (2*omega)*sin(gphi*d2r) per point, host libm, the same
+// expression as momentum_u_kern.c:251 / momentum_v_kern.c:230.
+inline std::vector<double> coriolis(const double *gphi, int n, double omega, double d2r)
+{
+    std::vector<double> f(n);
+    for (int k = 0; k < n; k++) f[k] = 2. * omega * std::sin(gphi[k] * d2r);
+    return f;
+}
+
+// The fused form copies nothing; it shares one buffer per after/now pair.
+// That reproduces every value the reference computes as long as no later
+// sweep reads a now-field outside the reference's copy region
+// (time_step.cpp:115-122). The only such read is momentum_u's un(j+1) on
+// the last T row, guarded by a positive tmask there.
+inline bool fused_allowed(const int *tm, const Geometry &G)
+{
+    const int jl = G.ye + 1;
+    for (int ji = 0; ji < G.W; ji++)
+        if (tm[(long)jl * G.W + ji] > 0) return false;
+    return true;
+}
+
+}  // namespace nemo

@@ -470,7 +470,7 @@ __global__ __launch_bounds__(64, MINW) void k_fused_wave(
     const int ci = min(max(i, 0), xmax);                 // clamped load column
     const int jlo = b.ys - 1 + band * R;
     const int jhi = min(jlo + R, ymax + 1);
-    auto rowoff = [&](int j) { return (long)min(max(j, 0), ymax) * W; };
+    auto rowoff = [&](int j) { return (unsigned)(min(max(j, 0), ymax) * W); };
     const T tide = tide_tab ? tide_tab[tide_k] : tide_val;
 
     // own-column window: 3-slot rows j-1, j, j+1; 2-slot rows j, j+1;
@@ -480,23 +480,23 @@ __global__ __launch_bounds__(64, MINW) void k_fused_wave(
     int tm[4];
 
     auto load3 = [&](int j, int s) {
-        const long c = rowoff(j) + ci;
-        un[s] = f.un[c];
-        vn[s] = f.vn[c];
-        hu[s] = f.hu[c];
-        su[s] = f.sshn_u[c];
-        hv[s] = f.hv[c];
-        sv[s] = f.sshn_v[c];
-        e1v[s] = f.dx_v[c];
-        e2u[s] = f.dy_u[c];
+        const unsigned c = rowoff(j) + ci;
+        un[s] = ld32(f.un, c);
+        vn[s] = ld32(f.vn, c);
+        hu[s] = ld32(f.hu, c);
+        su[s] = ld32(f.sshn_u, c);
+        hv[s] = ld32(f.hv, c);
+        sv[s] = ld32(f.sshn_v, c);
+        e1v[s] = ld32(f.dx_v, c);
+        e2u[s] = ld32(f.dy_u, c);
     };
     auto load2 = [&](int j, int s) {
-        const long c = rowoff(j) + ci;
-        ht[s] = f.ht[c];
-        st[s] = f.sshn_t[c];
-        e1t[s] = f.dx_t[c];
-        e2t[s] = f.dy_t[c];
-        at[s] = f.area_t[c];
+        const unsigned c = rowoff(j) + ci;
+        ht[s] = ld32(f.ht, c);
+        st[s] = ld32(f.sshn_t, c);
+        e1t[s] = ld32(f.dx_t, c);
+        e2t[s] = ld32(f.dy_t, c);
+        at[s] = ld32(f.area_t, c);
     };
     // new ssh of T(i, r): 3-slot s (row r) and s-1 (row r-1), 2-slot s2,
     // tm slots t-1..t+1; every lane evaluates it for its own column
@@ -525,18 +525,18 @@ __global__ __launch_bounds__(64, MINW) void k_fused_wave(
     load3(jlo - 1, 0);
     load3(jlo, 1);
     load2(jlo, 0);
-    tm[0] = f.tmask[rowoff(jlo - 1) + ci];
-    tm[1] = f.tmask[rowoff(jlo) + ci];
-    tm[2] = f.tmask[rowoff(jlo + 1) + ci];
+    tm[0] = ld32(f.tmask, rowoff(jlo - 1) + ci);
+    tm[1] = ld32(f.tmask, rowoff(jlo) + ci);
+    tm[2] = ld32(f.tmask, rowoff(jlo + 1) + ci);
     T ssh_c = new_ssh(1, 0, 1);  // T(i, jlo)
 
     for (int j = jlo; j < jhi; j++) {
         load3(j + 1, 2);
         load2(j + 1, 1);
-        tm[3] = f.tmask[rowoff(j + 2) + ci];
-        const long c = rowoff(j) + ci;
-        const T dxu = f.dx_u[c], dyv = f.dy_v[c], fcu = f.fcor_u[c], fcv = f.fcor_v[c];
-        const T aru = f.area_u[c], arv = f.area_v[c], sau = f.ssha_u[c], sav = f.ssha_v[c];
+        tm[3] = ld32(f.tmask, rowoff(j + 2) + ci);
+        const unsigned c = rowoff(j) + ci;
+        const T dxu = ld32(f.dx_u, c), dyv = ld32(f.dy_v, c), fcu = ld32(f.fcor_u, c), fcv = ld32(f.fcor_v, c);
+        const T aru = ld32(f.area_u, c), arv = ld32(f.area_v, c), sau = ld32(f.ssha_u, c), sav = ld32(f.ssha_v, c);
 
         const T ssh_n = new_ssh(2, 1, 2);   // T(i, j+1)
         const T ssh_e = from_above(ssh_c);  // T(i+1, j)
@@ -546,7 +546,7 @@ __global__ __launch_bounds__(64, MINW) void k_fused_wave(
         // takes its neighbour values right before use, which keeps them
         // short-lived.
         const bool own_t = owner && in_rect(i, j, b.xs, b.xe, b.ys, b.ye);
-        if (own_t) o.ssh[c] = ssh_c;
+        if (own_t) st32(o.ssh, c, ssh_c);
 
         const int tm_e0 = from_above(tm[1]);
         const int tm_e1 = from_above(tm[2]);
@@ -608,7 +608,7 @@ __global__ __launch_bounds__(64, MINW) void k_fused_wave(
                 val = T(0);
                 wrote = true;
             }
-            if (owner && wrote) o.u[c] = val;
+            if (owner && wrote) st32(o.u, c, val);
         }
         {  // V(i,j): momentum_v, else bc_solid_v
             MomVVals<T> v;
@@ -668,17 +668,17 @@ __global__ __launch_bounds__(64, MINW) void k_fused_wave(
                 val = T(0);
                 wrote = true;
             }
-            if (owner && wrote) o.v[c] = val;
+            if (owner && wrote) st32(o.v, c, val);
         }
         {  // next_sshu(i,j), next_sshv(i,j)
             const T at_e0 = from_above(at[0]);
             T val;
             if (owner && in_rect(i, j, b.xs, b.xe - 1, b.ys, b.ye) &&
                 next_ssh_pt(ssh_c, ssh_e, at[0], at_e0, aru, tm[1], tm_e0, &val))
-                o.sshu[c] = val;
+                st32(o.sshu, c, val);
             if (owner && in_rect(i, j, b.xs, b.xe, b.ys, b.ye - 1) &&
                 next_ssh_pt(ssh_c, ssh_n, at[0], at[1], arv, tm[1], tm[2], &val))
-                o.sshv[c] = val;
+                st32(o.sshv, c, val);
         }
 
 #define NEMO_SHIFT3(a) \

@@ -1,0 +1,49 @@
+// Mixed-precision step kernels (nemo_mixed.hip): FP32 fields, compensated
+// (hi, lo) FP32 ssh with FP64 accumulation.
+#pragma once
+
+#include "nemo_step.h"
+
+namespace nemo {
+
+struct MixedIn {
+    const float *__restrict__ ssh_hi;  // sshn_t, high word
+    const float *__restrict__ ssh_lo;  // sshn_t, low word
+    const float *__restrict__ sshn_u;
+    const float *__restrict__ sshn_v;
+    const float *__restrict__ un;
+    const float *__restrict__ vn;
+    const float *__restrict__ ht;
+    const float *__restrict__ hu;
+    const float *__restrict__ hv;
+    const float *__restrict__ ssha_u;
+    const float *__restrict__ ssha_v;
+    const float *__restrict__ area_t;
+    const float *__restrict__ area_u;
+    const float *__restrict__ area_v;
+    const float *__restrict__ dx_u;
+    const float *__restrict__ dx_v;
+    const float *__restrict__ dx_t;
+    const float *__restrict__ dy_u;
+    const float *__restrict__ dy_v;
+    const float *__restrict__ dy_t;
+    const float *__restrict__ fcor_u;
+    const float *__restrict__ fcor_v;
+    const int *__restrict__ tmask;
+};
+
+struct MixedOut {
+    float *ssh_hi, *ssh_lo, *u, *v, *sshu, *sshv;
+};
+
+// rows per band: 16, 32, 64 or 128
+void launch_mixed_step(hipStream_t s, int rows, const MixedIn &f,
+                       const StepScalars<float> &p, const MixedOut &o, int W,
+                       Bounds b, const double *tide_tab, int tide_k, double tide_val);
+void launch_flather_f32(hipStream_t s, const FlatherList &lu, const FlatherList &lv,
+                        float *ua, float *va, const float *su, const float *sv);
+void launch_to_f32(hipStream_t s, const double *src, float *dst, long n);
+void launch_split_f64(hipStream_t s, const double *src, float *hi, float *lo, long n);
+void launch_to_f64(hipStream_t s, const float *src, const float *lo, double *dst, long n);
+
+}  // namespace nemo

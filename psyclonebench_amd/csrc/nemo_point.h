@@ -58,6 +58,37 @@ struct StepScalars {
     T fric;  // 1 + cbfr*rdt
 };
 
+// Load / store at element offset `off` from a kernel-argument base pointer,
+// with the byte offset formed in 32 bits: the compiler then addresses with
+// the uniform base in SGPRs plus a 32-bit VGPR offset instead of a 64-bit
+// add per access. Every allocation here is far below 4 GiB.
+template <typename T>
+__device__ __forceinline__ T ld32(const T *__restrict__ base, unsigned off)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) +
+                                        (unsigned)(off * (unsigned)sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ void st32(T *__restrict__ base, unsigned off, T v)
+{
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) +
+                           (unsigned)(off * (unsigned)sizeof(T))) = v;
+}
+
+// Division used by the physics. FP64 keeps the IEEE quotient (the reference
+// rounding); FP32 (the mixed-precision form, which has an accuracy target
+// rather than a bitwise one) uses the hardware reciprocal, 2 ulp.
+template <typename T>
+__device__ __forceinline__ T divide(T a, T b)
+{
+    return a / b;
+}
+template <>
+__device__ __forceinline__ float divide<float>(float a, float b)
+{
+    return a * __builtin_amdgcn_rcpf(b);
+}
+
 template <typename T>
 __device__ __forceinline__ T upwind_hi(T x)
 {
@@ -89,7 +120,7 @@ __device__ __forceinline__ T continuity_val(const ContVals<T> &v, T rdt)
     const T flux_w = (v.su_w + v.hu_w) * v.un_w;
     const T flux_n = (v.sv_n + v.hv_n) * v.vn_n;
     const T flux_s = (v.sv_s + v.hv_s) * v.vn_s;
-    return v.sshn + (flux_w - flux_e + flux_s - flux_n) * rdt / v.area;
+    return v.sshn + divide((flux_w - flux_e + flux_s - flux_n) * rdt, v.area);
 }
 
 template <typename T>
@@ -182,7 +213,7 @@ __device__ __forceinline__ bool momentum_u_val(const MomUVals<T> &v,
         shear_s = T(0);
     } else {
         up_s = upwind_lo(v_s) * v.un_c + upwind_hi(v_s) * v.un_s;
-        shear_s = (v.un_c - v.un_s) / (v.e2u_c + v.e2u_s) *
+        shear_s = divide(v.un_c - v.un_s, v.e2u_c + v.e2u_s) *
                   (v.hu_c + v.su_c + v.hu_s + v.su_s);
     }
     if (shut_n) {
@@ -190,15 +221,15 @@ __device__ __forceinline__ bool momentum_u_val(const MomUVals<T> &v,
         shear_n = T(0);
     } else {
         up_n = upwind_hi(v_n) * v.un_c + upwind_lo(v_n) * v.un_n;
-        shear_n = (v.un_n - v.un_c) / (v.e2u_c + v.e2u_n) *
+        shear_n = divide(v.un_n - v.un_c, v.e2u_c + v.e2u_n) *
                   (v.hu_c + v.su_c + v.hu_n + v.su_n);
     }
     const T adv = up_w * u_w * dep_w - up_e * u_e * dep_e + up_s * v_s * dep_s -
                   up_n * v_n * dep_n;
 
     // viscosity (slip walls zero the cross-face shear above)
-    const T strain_e = (v.un_e - v.un_c) / v.e1t_e * (v.ht_e + v.st_e);
-    const T strain_w = (v.un_c - v.un_w) / v.e1t_c * (v.ht_c + v.st_c);
+    const T strain_e = divide(v.un_e - v.un_c, v.e1t_e) * (v.ht_e + v.st_e);
+    const T strain_w = divide(v.un_c - v.un_w, v.e1t_c) * (v.ht_c + v.st_c);
     T vis = (strain_e - strain_w) * v.e2u_c +
             (shear_n - shear_s) * v.e1u_c * T(0.5);
     vis = p.visc * vis;
@@ -207,8 +238,10 @@ __device__ __forceinline__ bool momentum_u_val(const MomUVals<T> &v,
                   (v.hu_c + v.su_c);
     const T hpg = -p.g * (v.hu_c + v.su_c) * v.e2u_c * (v.st_e - v.st_c);
 
-    *out = (v.un_c * (v.hu_c + v.su_c) + p.rdt * (adv + vis + cor + hpg) / v.area) /
-           (v.hu_c + v.ssha) / p.fric;
+    *out = divide(divide(v.un_c * (v.hu_c + v.su_c) +
+                             divide(p.rdt * (adv + vis + cor + hpg), v.area),
+                         v.hu_c + v.ssha),
+                  p.fric);
     return true;
 }
 
@@ -324,7 +357,7 @@ __device__ __forceinline__ bool momentum_v_val(const MomVVals<T> &v,
         shear_w = T(0);
     } else {
         up_w = upwind_lo(u_w) * v.vn_c + upwind_hi(u_w) * v.vn_w;
-        shear_w = (v.vn_c - v.vn_w) / (v.e1v_c + v.e1v_w) *
+        shear_w = divide(v.vn_c - v.vn_w, v.e1v_c + v.e1v_w) *
                   (v.hv_c + v.sv_c + v.hv_w + v.sv_w);
     }
     if (shut_e) {
@@ -332,14 +365,14 @@ __device__ __forceinline__ bool momentum_v_val(const MomVVals<T> &v,
         shear_e = T(0);
     } else {
         up_e = upwind_hi(u_e) * v.vn_c + upwind_lo(u_e) * v.vn_e;
-        shear_e = (v.vn_e - v.vn_c) / (v.e1v_c + v.e1v_e) *
+        shear_e = divide(v.vn_e - v.vn_c, v.e1v_c + v.e1v_e) *
                   (v.hv_c + v.sv_c + v.hv_e + v.sv_e);
     }
     const T adv = up_w * u_w * dep_w - up_e * u_e * dep_e + up_s * v_s * dep_s -
                   up_n * v_n * dep_n;
 
-    const T strain_n = (v.vn_n - v.vn_c) / v.e2t_n * (v.ht_n + v.st_n);
-    const T strain_s = (v.vn_c - v.vn_s) / v.e2t_c * (v.ht_c + v.st_c);
+    const T strain_n = divide(v.vn_n - v.vn_c, v.e2t_n) * (v.ht_n + v.st_n);
+    const T strain_s = divide(v.vn_c - v.vn_s, v.e2t_c) * (v.ht_c + v.st_c);
     T vis = (strain_n - strain_s) * v.e1v_c +
             (shear_e - shear_w) * v.e2v_c * T(0.5);
     vis = p.visc * vis;
@@ -348,8 +381,10 @@ __device__ __forceinline__ bool momentum_v_val(const MomVVals<T> &v,
                   (v.hv_c + v.sv_c);
     const T hpg = -p.g * (v.hv_c + v.sv_c) * v.e1v_c * (v.st_n - v.st_c);
 
-    *out = (v.vn_c * (v.hv_c + v.sv_c) + p.rdt * (adv + vis + cor + hpg) / v.area) /
-           (v.hv_c + v.ssha) / p.fric;
+    *out = divide(divide(v.vn_c * (v.hv_c + v.sv_c) +
+                             divide(p.rdt * (adv + vis + cor + hpg), v.area),
+                         v.hv_c + v.ssha),
+                  p.fric);
     return true;
 }
 
@@ -426,7 +461,7 @@ __device__ __forceinline__ bool next_ssh_pt(T ssh_c, T ssh_p, T area_c,
     if (m_c + m_p <= 0) return false;
     if (m_c * m_p > 0) {
         const T vol = area_c * ssh_c + area_p * ssh_p;
-        *out = T(0.5) * vol / area_face;
+        *out = divide(T(0.5) * vol, area_face);
     } else if (m_c <= 0) {
         *out = ssh_p;
     } else {

@@ -28,6 +28,9 @@ def main() -> None:
     p.add_argument("--unfused", type=int, default=2)
     p.add_argument("--variant", type=int, default=0,
                    help="fused kernel shape (-1 tile, 0.. row march)")
+    p.add_argument("--mixed", type=int, default=0,
+                   help="steps of the FP32 mixed form (plain launches)")
+    p.add_argument("--mixed-rows", type=int, default=64)
     a = p.parse_args()
     lib = _native.hip_lib()
     lib.nemo_hip_set_fused_variant(a.variant)
@@ -56,6 +59,13 @@ def main() -> None:
         cols = g.xe - g.xs + 3          # xs-1 .. xe+1
         out["copy3_bytes_read"] = 3 * 8 * rows * cols
         out["copy3_bytes_written"] = 3 * 8 * rows * cols
+    if a.mixed:
+        from .psy import MixedStepper
+        st = MixedStepper(Model(cfg), rows=a.mixed_rows)
+        st.run(1)
+        out["mixed_kernel_ms"] = st.timed_run(a.mixed)
+        st.release()
+        out["mixed_alg_bytes_per_launch"] = 116 * (cfg.jpiglo - 2) * (cfg.jpjglo - 2)
     print(json.dumps(out), flush=True)
 
 

@@ -117,6 +117,48 @@ class TimeStepper:
             f.device_ptr.value = None
 
 
+class MixedStepper:
+    """The mixed-precision form (nemo_mp_*): FP32 fields, compensated
+    hi/lo FP32 ssh with FP64 accumulation, FP64 tide. Starts from the
+    model's FP64 host arrays."""
+
+    FIELDS = {"sshn_t": 0, "un": 1, "vn": 2, "sshn_u": 3, "sshn_v": 4}
+
+    def __init__(self, model: Model, rows: int = 32):
+        lib = hip_lib()
+        g = model.grid
+        c = model.cfg
+        self.model = model
+        self.istp = c.nit000
+        self.handle = lib.nemo_mp_create(*model.host_pointers(), g.xs, g.xe, g.ys, g.ye,
+                                         g.W, g.total, *model.scalars, c.tide_amp,
+                                         c.tide_omega)
+        lib.nemo_mp_set_rows(self.handle, rows)
+
+    def run(self, nsteps: int) -> None:
+        hip_lib().nemo_mp_steps(self.handle, nsteps, self.istp)
+        self.istp += nsteps
+
+    def timed_run(self, nsteps: int) -> float:
+        """nsteps event-timed steps; mean kernel ms."""
+        ms = hip_lib().nemo_mp_kernel_ms(self.handle, nsteps, self.istp)
+        self.istp += nsteps
+        return ms
+
+    def fetch(self, name: str):
+        import numpy as np
+        g = self.model.grid
+        out = np.empty((g.H, g.W), np.float64)
+        hip_lib().nemo_mp_read(self.handle, self.FIELDS[name],
+                               out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def release(self) -> None:
+        if self.handle:
+            hip_lib().nemo_mp_destroy(self.handle)
+            self.handle = None
+
+
 def invoke_time_step_host(model: Model, istp: int) -> None:
     """Host-pointer step (psykal_cpp/time_step.cpp:10-51) through
     c_invoke_time_step_host: host arrays in, host arrays updated."""
