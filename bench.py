@@ -36,7 +36,7 @@ METRIC = ("grid-pt updates/s + % HBM peak at 4096²; "
 HBM_PEAK_GBS = 8000.0
 B_ALG_F64 = 212          # bytes per interior grid-point update, FP64
 B_ALG_MIXED = 116        # FP32 fields + the ssh compensation word
-PROFILE_TRAFFIC = ROOT / "profiles" / "fused_step_f64_4096_traffic.txt"
+PROFILE_TRAFFIC = ROOT / "profiles" / "step_kernels_4096_traffic.md"
 
 
 def parse_args():
@@ -56,14 +56,16 @@ def parse_args():
     return p.parse_args()
 
 
-def traffic_from_profile():
-    """HBM bytes per fused-step launch measured with rocprofv3 PMC counters
-    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), as
-    recorded in profiles/; None when not measured."""
+def traffic_from_profile(label: str):
+    """HBM bytes per launch of the kernel recorded under `label` ("variant 13",
+    "mixed rows 64"), measured with rocprofv3 PMC counters (FETCH_SIZE +
+    WRITE_SIZE, scaled by the copy-kernel calibration) and kept in
+    profiles/; None when that kernel shape was not profiled."""
     if not PROFILE_TRAFFIC.exists():
         return None
+    key = f"[{label}] traffic_bytes_per_launch:"
     for line in PROFILE_TRAFFIC.read_text().splitlines():
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.strip().startswith(key):
             return float(line.split(":", 1)[1])
     return None
 
@@ -167,6 +169,7 @@ def mixed_leg(pristine, a, pts, f64_ms_per_step, f64_kernel_ms, f64_variant) -> 
         "rows_kernel_ms": {str(k): round(v, 4) for k, v in tune.items()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic_from_profile(f"mixed rows {rows}"),
                      "kernel": "k_mixed_wave", "kernel_ms": kernel_ms,
                      "bytes_per_launch_alg": b_alg},
         "error_vs_f64": {"metric": "(relmax, relrms) over the internal T region",
@@ -229,7 +232,7 @@ def main():
     st.release()
     bytes_alg = B_ALG_F64 * pts
     achieved = bytes_alg / (kernel_ms * 1e-3) / 1e9
-    traffic = traffic_from_profile()
+    traffic = traffic_from_profile(f"variant {best}")
 
     out = {
         "metric": METRIC,
@@ -254,7 +257,8 @@ def main():
             "hbm_frac_whole_step": B_ALG_F64 * updates_per_s / (HBM_PEAK_GBS * 1e9),
             "fused_variant": best,
             "fused_shape": {"block_columns": bw.value, "rows": rows.value,
-                            "prefetch": bool(pf.value)},
+                            "kind": ["row march", "row march, loads one row ahead",
+                                     "overlapped-wavefront march"][min(pf.value, 2)]},
             "variant_kernel_ms": {str(k): round(v, 4) for k, v in tune.items()},
         },
         "roofline": {
@@ -264,7 +268,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": "k_fused_step",
+            "kernel": lib.nemo_hip_fused_variant_kernel(best).decode(),
             "kernel_ms": kernel_ms,
             "bytes_per_launch_alg": bytes_alg,
         },

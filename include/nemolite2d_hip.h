@@ -148,10 +148,14 @@ NEMO_API double nemo_hip_fused_kernel_ms(void);
 NEMO_API int nemo_hip_fused_variant_count(void);
 NEMO_API void nemo_hip_set_fused_variant(int variant);
 NEMO_API int nemo_hip_fused_variant(void);
-/* Block width (columns), rows per block, and whether the row loads are
- * issued one row ahead (1) of a variant. */
+/* Block width (columns), rows per block, and the march kind of a variant:
+ * 0 = row march, 1 = row march with the row loads issued one row ahead,
+ * 2/3/4 = overlapped-wavefront march with at least 1/3/4 waves per SIMD
+ * requested from the register allocator. */
 NEMO_API void nemo_hip_fused_variant_shape(int variant, int *block_width,
                                            int *rows, int *prefetch);
+/* Name of the kernel a variant launches, as profilers report it. */
+NEMO_API const char *nemo_hip_fused_variant_kernel(int variant);
 
 /* ---- mixed-precision form (no reference counterpart) -------------------
  * FP32 storage and arithmetic for velocities, face ssh, depths and metrics;

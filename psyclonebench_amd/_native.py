@@ -85,6 +85,8 @@ def hip_lib() -> C.CDLL:
         lib.nemo_hip_fused_variant.restype = C.c_int
         lib.nemo_hip_fused_variant_count.restype = C.c_int
         lib.nemo_hip_fused_variant_shape.argtypes = [C.c_int, _ip, _ip, _ip]
+        lib.nemo_hip_fused_variant_kernel.argtypes = [C.c_int]
+        lib.nemo_hip_fused_variant_kernel.restype = C.c_char_p
         lib.nemo_mp_create.argtypes = _HOST + [C.c_int] * 6 + [C.c_double] * 8
         lib.nemo_mp_create.restype = C.c_void_p
         lib.nemo_mp_steps.argtypes = [C.c_void_p, C.c_int, C.c_int]

@@ -741,7 +741,31 @@ static const MarchShape kShapes[] = {
     {64, 64, 4},
 };
 
+// Kernel names as the profiler reports them (template arguments included).
+static const char *const kNames[] = {
+    "k_fused_march<256, 32, false>", "k_fused_march<256, 64, false>",
+    "k_fused_march<256, 16, false>", "k_fused_march<128, 32, false>",
+    "k_fused_march<128, 64, false>", "k_fused_march<64, 64, false>",
+    "k_fused_march<64, 128, false>", "k_fused_march<256, 32, true>",
+    "k_fused_march<256, 64, true>",  "k_fused_march<256, 16, true>",
+    "k_fused_march<128, 32, true>",  "k_fused_march<64, 64, true>",
+    "k_fused_march<64, 32, true>",   "k_fused_wave<16, 1>",
+    "k_fused_wave<32, 1>",           "k_fused_wave<64, 1>",
+    "k_fused_wave<128, 1>",          "k_fused_wave<32, 3>",
+    "k_fused_wave<64, 3>",           "k_fused_wave<32, 4>",
+    "k_fused_wave<64, 4>",
+};
+static_assert(sizeof(kNames) / sizeof(kNames[0]) == sizeof(kShapes) / sizeof(kShapes[0]),
+              "one kernel name per shape");
+
 int march_variant_count() { return (int)(sizeof(kShapes) / sizeof(kShapes[0])); }
+
+const char *march_variant_kernel(int variant)
+{
+    if (variant < 0 || variant >= march_variant_count())
+        NEMO_FATAL("unknown fused march variant %d", variant);
+    return kNames[variant];
+}
 
 void march_variant_shape(int variant, int *mx, int *rows, int *prefetch)
 {

@@ -508,6 +508,11 @@ void nemo_hip_fused_variant_shape(int variant, int *block_width, int *rows,
     march_variant_shape(variant, block_width, rows, prefetch);
 }
 
+const char *nemo_hip_fused_variant_kernel(int variant)
+{
+    return variant == -1 ? "k_fused_step" : march_variant_kernel(variant);
+}
+
 double nemo_hip_fused_kernel_ms(void) { return g_fused_ms; }
 
 void nemo_hip_release(void **ssha_t_device)

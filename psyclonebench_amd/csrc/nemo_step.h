@@ -85,6 +85,7 @@ void launch_fused_march(hipStream_t s, int variant, const StepIn<double> &f,
                         double tide_val);
 int march_variant_count();
 void march_variant_shape(int variant, int *mx, int *rows, int *prefetch);
+const char *march_variant_kernel(int variant);
 
 // Small helpers.
 void launch_fill_int(hipStream_t s, int *p, int n, int value);

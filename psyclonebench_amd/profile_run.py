@@ -26,25 +26,29 @@ def main() -> None:
     p.add_argument("--n", type=int, default=4096)
     p.add_argument("--fused", type=int, default=10)
     p.add_argument("--unfused", type=int, default=2)
-    p.add_argument("--variant", type=int, default=0,
-                   help="fused kernel shape (-1 tile, 0.. row march)")
+    p.add_argument("--variant", type=str, default="0",
+                   help="fused kernel shape(s), comma separated (-1 tile, 0.. march)")
     p.add_argument("--mixed", type=int, default=0,
                    help="steps of the FP32 mixed form (plain launches)")
-    p.add_argument("--mixed-rows", type=int, default=64)
+    p.add_argument("--mixed-rows", type=str, default="64",
+                   help="rows per band of the mixed kernel, comma separated")
     a = p.parse_args()
     lib = _native.hip_lib()
-    lib.nemo_hip_set_fused_variant(a.variant)
+    variants = [int(v) for v in a.variant.split(",")]
     cfg = c2_config(a.n, 1, seed=0)
-    out = {"n": a.n, "variant": a.variant}
+    out = {"n": a.n, "variants": variants}
     if a.fused:
-        m = Model(cfg)
-        g = m.grid
-        st = TimeStepper(m, mode=2)
-        lib.nemo_hip_set_kernel_timing(1)      # plain launches, event-timed
-        st.run(a.fused + 1)
-        out["fused_kernel_ms"] = lib.nemo_hip_fused_kernel_ms()
-        lib.nemo_hip_set_kernel_timing(0)
-        st.release()
+        out["fused_kernel_ms"] = {}
+        for v in variants:
+            lib.nemo_hip_set_fused_variant(v)
+            m = Model(cfg)
+            g = m.grid
+            st = TimeStepper(m, mode=2)
+            lib.nemo_hip_set_kernel_timing(1)      # plain launches, event-timed
+            st.run(a.fused + 1)
+            out["fused_kernel_ms"][v] = lib.nemo_hip_fused_kernel_ms()
+            lib.nemo_hip_set_kernel_timing(0)
+            st.release()
         pts = (g.M - 2) * (g.N - 2)
         out["fused_alg_bytes_per_launch"] = B_ALG_F64 * pts
     if a.unfused:
@@ -61,10 +65,12 @@ def main() -> None:
         out["copy3_bytes_written"] = 3 * 8 * rows * cols
     if a.mixed:
         from .psy import MixedStepper
-        st = MixedStepper(Model(cfg), rows=a.mixed_rows)
-        st.run(1)
-        out["mixed_kernel_ms"] = st.timed_run(a.mixed)
-        st.release()
+        out["mixed_kernel_ms"] = {}
+        for rows in [int(r) for r in a.mixed_rows.split(",")]:
+            st = MixedStepper(Model(cfg), rows=rows)
+            st.run(1)
+            out["mixed_kernel_ms"][rows] = st.timed_run(a.mixed)
+            st.release()
         out["mixed_alg_bytes_per_launch"] = 116 * (cfg.jpiglo - 2) * (cfg.jpjglo - 2)
     print(json.dumps(out), flush=True)
 

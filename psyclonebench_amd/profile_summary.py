@@ -2,7 +2,12 @@
 the short text kept under profiles/.
 
     python -m psyclonebench_amd.profile_summary --trace T.db --fetch F.db \
-        --write W.db --kernel k_fused_march --alg-bytes 3556769792
+        --write W.db --report "variant 13=k_fused_wave<16, 1>=3556769792" \
+        --copy-bytes 402948144
+
+Each --report names a label, a substring of the kernel name and the
+algorithmic bytes of one launch; the summary prints lines
+"[label] traffic_bytes_per_launch: B" that the benchmark reads back.
 
 Calibration: k_copy3 (the unfused path's now<-after copy) reads and writes a
 known number of bytes; FETCH_SIZE and WRITE_SIZE of every kernel are scaled
@@ -40,16 +45,16 @@ def main() -> None:
     p.add_argument("--trace", required=True)
     p.add_argument("--fetch", required=True)
     p.add_argument("--write", required=True)
-    p.add_argument("--kernel", required=True, help="substring of the kernel to report")
-    p.add_argument("--alg-bytes", type=float, required=True,
-                   help="algorithmic bytes per launch of that kernel")
+    p.add_argument("--report", action="append", required=True,
+                   help="LABEL=KERNEL_SUBSTRING=ALG_BYTES_PER_LAUNCH (repeatable)")
     p.add_argument("--copy-bytes", type=float, required=True,
                    help="bytes k_copy3 reads (= writes) per launch")
     a = p.parse_args()
 
+    stats = kernel_stats(a.trace)
     print("## kernel trace (rocprofv3 --kernel-trace --stats)")
     print(f"{'kernel':<40} {'calls':>6} {'avg_us':>10} {'total_ms':>10}")
-    for name, calls, total, avg in kernel_stats(a.trace):
+    for name, calls, total, avg in stats:
         print(f"{_short(name)[:40]:<40} {calls:>6} {avg:>10.2f} {total / 1e3:>10.3f}")
 
     fetch = counter_means(a.fetch, "FETCH_SIZE")
@@ -67,17 +72,21 @@ def main() -> None:
         f = fetch[k][1] * kf
         w = write.get(k, (0, 0.0, 0.0))[1] * kw
         print(f"{_short(k)[:40]:<40} {f / 1e6:>10.1f} {w / 1e6:>10.1f} {fetch[k][2] / 1e3:>10.1f}")
-    key = [k for k in fetch if a.kernel in k][0]
-    traffic = fetch[key][1] * kf + write[key][1] * kw
-    dur_ns = 1e3 * [avg for name, _, _, avg in kernel_stats(a.trace) if a.kernel in name][0]
-    print()
-    print(f"kernel: {_short(key)}")
-    print(f"algorithmic_bytes_per_launch: {a.alg_bytes:.0f}")
-    print(f"traffic_bytes_per_launch: {traffic:.0f}")
-    print(f"traffic_over_algorithmic: {traffic / a.alg_bytes:.3f}")
-    print(f"avg_duration_us (kernel trace): {dur_ns / 1e3:.2f}")
-    print(f"achieved_algorithmic_GBps: {a.alg_bytes / dur_ns:.1f}")
-    print(f"achieved_traffic_GBps: {traffic / dur_ns:.1f}")
+
+    for rep in a.report:
+        label, sub, alg = rep.split("=")
+        alg_bytes = float(alg)
+        key = [k for k in fetch if sub in k][0]
+        traffic = fetch[key][1] * kf + write[key][1] * kw
+        dur_us = [avg for name, _, _, avg in stats if sub in name][0]
+        print()
+        print(f"[{label}] kernel: {_short(key)}")
+        print(f"[{label}] algorithmic_bytes_per_launch: {alg_bytes:.0f}")
+        print(f"[{label}] traffic_bytes_per_launch: {traffic:.0f}")
+        print(f"[{label}] traffic_over_algorithmic: {traffic / alg_bytes:.3f}")
+        print(f"[{label}] avg_duration_us (kernel trace): {dur_us:.2f}")
+        print(f"[{label}] achieved_algorithmic_GBps: {alg_bytes / (dur_us * 1e3):.1f}")
+        print(f"[{label}] achieved_traffic_GBps: {traffic / (dur_us * 1e3):.1f}")
 
 
 if __name__ == "__main__":
