@@ -53,6 +53,9 @@ def parse_args():
                    help="target CPU time of the cpu_baseline sample")
     p.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     p.add_argument("--no-mixed", action="store_true", help="skip the FP32 mixed leg")
+    p.add_argument("--no-ensemble", action="store_true", help="skip the ensemble leg")
+    p.add_argument("--ens-domains", type=int, default=256)
+    p.add_argument("--ens-steps", type=int, default=2000)
     return p.parse_args()
 
 
@@ -177,6 +180,68 @@ def mixed_leg(pristine, a, pts, f64_ms_per_step, f64_kernel_ms, f64_variant) -> 
     }
 
 
+def ensemble_leg(a) -> dict:
+    """The batched ensemble: --ens-domains domains (seed 0, sides
+    log-uniform in [64, 1024], own bathymetry and tide), --ens-steps steps
+    per precision in one graph replay after an autotune of the persistent
+    launch shape; kernel roofline from HIP events; FP32 (mixed form) error
+    against FP64 after the same steps, worst domain per field."""
+    from psyclonebench_amd.ensemble import EnsembleStepper, ensemble_configs
+    from psyclonebench_amd.model import Model
+
+    cfgs = ensemble_configs(a.ens_domains, seed=0, nsteps=a.ens_steps)
+    models = [Model(c) for c in cfgs]
+    out = {"workload": (f"{a.ens_domains} domains, seed 0, interior sides log-uniform in "
+                        "[64,1024], mean depth U[50,500] m +-50% relief, tide amplitude "
+                        "U[0.5,2] m, period U[40000,46000] s, open west edge"),
+           "steps": a.ens_steps}
+    fields = ("sshn_t", "un", "vn")
+    finals = {}
+    for prec, b_alg in ((64, B_ALG_F64), (32, B_ALG_MIXED)):
+        ens = EnsembleStepper(models, precision=prec)
+        pts = ens.interior_points
+        best, tune_ms = ens.autotune(5)
+        shape = ens.shapes()[best]
+        ens.run(1 + a.warmup)
+        ens.synchronize()
+        t0 = time.perf_counter()
+        ens.run(a.ens_steps)
+        ens.synchronize()
+        wall = time.perf_counter() - t0
+        kernel_ms = ens.timed_run(min(a.kernel_steps, 100))
+        achieved = b_alg * pts / (kernel_ms * 1e-3) / 1e9
+        out[f"f{prec}"] = {
+            "value": pts * a.ens_steps / wall,
+            "unit": "grid-pt updates/s",
+            "interior_points": pts,
+            "ms_per_step": wall * 1e3 / a.ens_steps,
+            "shape": {"rows": shape[0], "waves_per_workgroup": shape[1],
+                      "workgroups": shape[2]},
+            "autotune_kernel_ms": [round(x, 4) for x in tune_ms],
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                         "kernel": f"k_ens_{'f64' if prec == 64 else 'mixed'}<{shape[0]}>",
+                         "kernel_ms": kernel_ms, "bytes_per_launch_alg": b_alg * pts},
+        }
+        ens.release()
+        # accuracy run: the same number of steps from the initial state
+        ens = EnsembleStepper(models, precision=prec)
+        ens.set_shape(best)
+        ens.run(a.ens_steps)
+        finals[prec] = [{n: ens.fetch(d, n)[m.grid.ys:m.grid.ye + 1, m.grid.xs:m.grid.xe + 1]
+                         for n in fields} for d, m in enumerate(models)]
+        ens.release()
+    worst = {n: [0.0, 0.0] for n in fields}
+    for d in range(len(models)):
+        for n in fields:
+            e = rel_err(finals[32][d][n], finals[64][d][n])
+            worst[n] = [max(worst[n][0], e[0]), max(worst[n][1], e[1])]
+    out["f32_error_vs_f64"] = {"metric": "(relmax, relrms) over the internal T region, "
+                                         "worst domain", "after_steps": a.ens_steps,
+                               "fields": worst}
+    return out
+
+
 def main():
     a = parse_args()
     if a.gpus != 1:
@@ -275,6 +340,8 @@ def main():
     }
     if not a.no_mixed:
         out["mixed"] = mixed_leg(pristine, a, pts, wall * 1e3 / a.steps, kernel_ms, best)
+    if not a.no_ensemble:
+        out["ensemble"] = ensemble_leg(a)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(a.n, a.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -182,6 +182,45 @@ NEMO_API void nemo_mp_set_rows(void *handle, int rows);
 NEMO_API void nemo_mp_read(void *handle, int field, double *out);
 NEMO_API void nemo_mp_destroy(void *handle);
 
+/* ---- batched ensemble (no reference counterpart) -----------------------
+ * ndom independent domains, each in its own dl_esm_inf layout, advanced
+ * together: one persistent launch per step walks a tile -> domain work list,
+ * then one launch runs every domain's Flather segments.
+ *   arrays      ndom x 25 host pointers, each row in the c_invoke_time_step
+ *               argument order (tmask as int*); read during creation only
+ *   geom        ndom x 6 ints: xs, xe, ys, ye (0-based internal bounds),
+ *               width, total_size
+ *   tide_amp, tide_omega   per domain: bc_ssh = amp*sin(omega*istep*rdt)
+ *   precision   64: the FP64 step (bitwise the single-domain fused form);
+ *               32: the mixed form (bitwise the single-domain nemo_mp_*)
+ * Every domain's mask must allow the fused form (no wet cell on row ye+1).
+ * One field's arena (every domain, offsets rounded to 64 elements) must stay
+ * below 4 GiB. Returns a handle; errors exit(1). */
+NEMO_API void *nemo_ens_create(int ndom, void *const *arrays, const int *geom,
+                               const double *tide_amp, const double *tide_omega,
+                               double rdt, double cbfr, double visc, double omega,
+                               double d2r, double g, int precision);
+/* Launch shapes: rows per band (tile height; tile width is 61 columns),
+ * wavefronts per workgroup, workgroups per launch. */
+NEMO_API int nemo_ens_shape_count(void *handle);
+NEMO_API void nemo_ens_shape_info(void *handle, int shape, int *rows,
+                                  int *waves_per_block, int *blocks);
+NEMO_API void nemo_ens_set_shape(void *handle, int shape);
+NEMO_API int nemo_ens_shape(void *handle);
+/* Times every shape on steps_per_shape steps of this ensemble (HIP events
+ * around each step kernel; mean ms per shape into ms_out if not NULL),
+ * restores the state, selects and returns the fastest shape. */
+NEMO_API int nemo_ens_autotune(void *handle, int steps_per_shape, double *ms_out);
+/* nsteps steps istep, istep+1, ... (one hipGraph replay; asynchronous). */
+NEMO_API void nemo_ens_steps(void *handle, int nsteps, int istep);
+/* The same as plain launches; returns the mean step-kernel time in ms. */
+NEMO_API double nemo_ens_kernel_ms(void *handle, int nsteps, int istep);
+NEMO_API long long nemo_ens_interior_points(void *handle);
+/* Blocking read-back of one domain as FP64 (its total_size values): field
+ * 0 sshn_t, 1 un, 2 vn, 3 sshn_u, 4 sshn_v. */
+NEMO_API void nemo_ens_read(void *handle, int domain, int field, double *out);
+NEMO_API void nemo_ens_destroy(void *handle);
+
 /* Releases the device buffers, graphs and lists of one context (identified
  * by its ssha_t slot), or of all contexts. Slots become dangling. */
 NEMO_API void nemo_hip_release(void **ssha_t_device);

@@ -95,6 +95,25 @@ def hip_lib() -> C.CDLL:
         lib.nemo_mp_set_rows.argtypes = [C.c_void_p, C.c_int]
         lib.nemo_mp_read.argtypes = [C.c_void_p, C.c_int, _dp]
         lib.nemo_mp_destroy.argtypes = [C.c_void_p]
+        vp = C.c_void_p
+        lib.nemo_ens_create.argtypes = [C.c_int, C.POINTER(vp), _ip, _dp, _dp] + \
+            [C.c_double] * 6 + [C.c_int]
+        lib.nemo_ens_create.restype = vp
+        lib.nemo_ens_shape_count.argtypes = [vp]
+        lib.nemo_ens_shape_count.restype = C.c_int
+        lib.nemo_ens_shape_info.argtypes = [vp, C.c_int, _ip, _ip, _ip]
+        lib.nemo_ens_set_shape.argtypes = [vp, C.c_int]
+        lib.nemo_ens_shape.argtypes = [vp]
+        lib.nemo_ens_shape.restype = C.c_int
+        lib.nemo_ens_autotune.argtypes = [vp, C.c_int, _dp]
+        lib.nemo_ens_autotune.restype = C.c_int
+        lib.nemo_ens_steps.argtypes = [vp, C.c_int, C.c_int]
+        lib.nemo_ens_kernel_ms.argtypes = [vp, C.c_int, C.c_int]
+        lib.nemo_ens_kernel_ms.restype = C.c_double
+        lib.nemo_ens_interior_points.argtypes = [vp]
+        lib.nemo_ens_interior_points.restype = C.c_longlong
+        lib.nemo_ens_read.argtypes = [vp, C.c_int, C.c_int, _dp]
+        lib.nemo_ens_destroy.argtypes = [vp]
         lib.nemo_hip_release.argtypes = [_vpp]
         lib.nemo_hip_device_info.argtypes = [C.c_char_p, C.c_int]
         lib.nemo_hip_device_info.restype = C.c_int

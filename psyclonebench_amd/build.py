@@ -38,7 +38,7 @@ HIPCC_FLAGS = [
 ]
 
 HIP_SOURCES = ["nemo_step.hip", "nemo_march.hip", "nemo_psy.cpp", "nemo_mixed.hip",
-               "nemo_mixed_api.cpp"]
+               "nemo_mixed_api.cpp", "nemo_ensemble.hip", "nemo_ensemble_api.cpp"]
 
 
 def _hipcc() -> str:

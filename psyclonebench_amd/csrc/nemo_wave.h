@@ -1,0 +1,565 @@
+// Overlapped-wavefront tile of the fused NEMOLite2D step, FP64 and mixed
+// precision: the bodies shared by the single-domain kernels (nemo_march.hip,
+// nemo_mixed.hip) and the ensemble kernels (nemo_ensemble.hip).
+//
+// Overlapped-wavefront march. One block = one wavefront of 64 lanes over 64
+// consecutive columns, of which lanes 1..61 own their column's outputs and
+// lanes 0, 62, 63 are halo (neighbouring strips overlap by 3 columns). Every
+// value a lane needs at column i-1 or i+1 sits in lane l-1 / l+1 of the same
+// wavefront, so it is taken with a cross-lane permute (ds_bpermute) rather
+// than loaded or kept in a second register: each lane carries only its own
+// column's window, which roughly halves the registers of the other march and
+// raises the wavefronts per SIMD. The new ssh of T(i+1) comes from lane l+1,
+// which computes it for its own column (lane 62 is the last one that needs
+// it, and its stencil ends at lane 63).
+//
+// A tile is one strip (61 output columns, 64 lanes) of one band (R rows) of
+// one domain; the caller picks the tile and passes the arrays, the element
+// offset `base` of the domain's point (0,0) in them, its pitch W and its
+// internal bounds. Element offsets times the element size must stay below
+// 2^32 (every access is a 32-bit byte offset from a uniform base).
+#pragma once
+
+#include "nemo_mixed.h"
+#include "nemo_step.h"
+
+namespace nemo {
+namespace wave {
+
+constexpr int OUT = 61;
+
+__device__ __forceinline__ bool in_rect(int i, int j, int x0, int x1, int y0,
+                                        int y1)
+{
+    return i >= x0 && i <= x1 && j >= y0 && j <= y1;
+}
+
+__device__ __forceinline__ double from_below(double x) { return __shfl_up(x, 1, 64); }
+__device__ __forceinline__ double from_above(double x) { return __shfl_down(x, 1, 64); }
+__device__ __forceinline__ float from_below(float x) { return __shfl_up(x, 1, 64); }
+__device__ __forceinline__ float from_above(float x) { return __shfl_down(x, 1, 64); }
+__device__ __forceinline__ int from_below(int x) { return __shfl_up(x, 1, 64); }
+__device__ __forceinline__ int from_above(int x) { return __shfl_down(x, 1, 64); }
+
+// Strips and bands of the extent the fused step walks: columns xs-1 .. xe+1,
+// rows ys-1 .. ye+1.
+__host__ __device__ inline int strips(const Bounds &b) { return (b.xe - b.xs + 3 + OUT - 1) / OUT; }
+__host__ __device__ inline int bands(const Bounds &b, int R) { return (b.ye - b.ys + 3 + R - 1) / R; }
+
+// Block b of a launch of nblk blocks -> logical tile, so that each of the 8
+// XCDs (the dispatcher hands block b to XCD b % 8) owns one contiguous range
+// of tiles, strips fastest.
+__device__ __forceinline__ int xcd_tile(int bid, int nblk)
+{
+    const int xcd = bid & 7, k = bid >> 3;
+    const int per = nblk >> 3, extra = nblk & 7;
+    return xcd * per + min(xcd, extra) + k;
+}
+
+// One FP64 tile: new ssh (continuity + bc_ssh), momentum + bc_solid for U
+// and V, next_sshu/v, stored where the lane owns its column.
+template <int R>
+__device__ __forceinline__ void tile_f64(const StepIn<double> &f,
+                                         const StepScalars<double> &p,
+                                         const StepOut<double> &o, int W,
+                                         unsigned base,
+                                         const Bounds &b, int strip, int band,
+                                         double tide)
+{
+    using T = double;
+    const int xmax = b.xe + 1, ymax = b.ye + 1;
+    const int lane = (int)(threadIdx.x & 63);  // lane within the wavefront
+    const int i = b.xs - 1 + strip * OUT + lane - 1;  // may be -1 or > xmax
+    const bool owner = lane >= 1 && lane <= OUT && i <= xmax;
+    const int ci = min(max(i, 0), xmax);                 // clamped load column
+    const int jlo = b.ys - 1 + band * R;
+    const int jhi = min(jlo + R, ymax + 1);
+    auto rowoff = [&](int j) { return base + (unsigned)(min(max(j, 0), ymax) * W); };
+
+    // own-column window: 3-slot rows j-1, j, j+1; 2-slot rows j, j+1;
+    // tm rows j-1 .. j+2
+    T un[3], vn[3], hu[3], su[3], hv[3], sv[3], e1v[3], e2u[3];
+    T ht[2], st[2], e1t[2], e2t[2], at[2];
+    int tm[4];
+
+    auto load3 = [&](int j, int s) {
+        const unsigned c = rowoff(j) + ci;
+        un[s] = ld32(f.un, c);
+        vn[s] = ld32(f.vn, c);
+        hu[s] = ld32(f.hu, c);
+        su[s] = ld32(f.sshn_u, c);
+        hv[s] = ld32(f.hv, c);
+        sv[s] = ld32(f.sshn_v, c);
+        e1v[s] = ld32(f.dx_v, c);
+        e2u[s] = ld32(f.dy_u, c);
+    };
+    auto load2 = [&](int j, int s) {
+        const unsigned c = rowoff(j) + ci;
+        ht[s] = ld32(f.ht, c);
+        st[s] = ld32(f.sshn_t, c);
+        e1t[s] = ld32(f.dx_t, c);
+        e2t[s] = ld32(f.dy_t, c);
+        at[s] = ld32(f.area_t, c);
+    };
+    // new ssh of T(i, r): 3-slot s (row r) and s-1 (row r-1), 2-slot s2,
+    // tm slots t-1..t+1; every lane evaluates it for its own column
+    auto new_ssh = [&](int s, int s2, int t) {
+        ContVals<T> v;
+        v.sshn = st[s2];
+        v.su_e = su[s];
+        v.hu_e = hu[s];
+        v.un_e = un[s];
+        v.su_w = from_below(su[s]);
+        v.hu_w = from_below(hu[s]);
+        v.un_w = from_below(un[s]);
+        v.sv_n = sv[s];
+        v.hv_n = hv[s];
+        v.vn_n = vn[s];
+        v.sv_s = sv[s - 1];
+        v.hv_s = hv[s - 1];
+        v.vn_s = vn[s - 1];
+        v.area = at[s2];
+        const int m_w = from_below(tm[t]), m_e = from_above(tm[t]);
+        const T x = continuity_val(v, p.rdt);
+        return tide_applies(tm[t], tm[t - 1], tm[t + 1], m_e, m_w) ? tide : x;
+    };
+
+    // ---- prologue ----
+    load3(jlo - 1, 0);
+    load3(jlo, 1);
+    load2(jlo, 0);
+    tm[0] = ld32(f.tmask, rowoff(jlo - 1) + ci);
+    tm[1] = ld32(f.tmask, rowoff(jlo) + ci);
+    tm[2] = ld32(f.tmask, rowoff(jlo + 1) + ci);
+    T ssh_c = new_ssh(1, 0, 1);  // T(i, jlo)
+
+    for (int j = jlo; j < jhi; j++) {
+        load3(j + 1, 2);
+        load2(j + 1, 1);
+        tm[3] = ld32(f.tmask, rowoff(j + 2) + ci);
+        const unsigned c = rowoff(j) + ci;
+        const T dxu = ld32(f.dx_u, c), dyv = ld32(f.dy_v, c), fcu = ld32(f.fcor_u, c), fcv = ld32(f.fcor_v, c);
+        const T aru = ld32(f.area_u, c), arv = ld32(f.area_v, c), sau = ld32(f.ssha_u, c), sav = ld32(f.ssha_v, c);
+
+        const T ssh_n = new_ssh(2, 1, 2);   // T(i, j+1)
+        const T ssh_e = from_above(ssh_c);  // T(i+1, j)
+
+        // Every lane runs every phase (the permutes are wavefront-wide);
+        // only the stores depend on the lane owning its column. Each phase
+        // takes its neighbour values right before use, which keeps them
+        // short-lived.
+        const bool own_t = owner && in_rect(i, j, b.xs, b.xe, b.ys, b.ye);
+        if (own_t) st32(o.ssh, c, ssh_c);
+
+        const int tm_e0 = from_above(tm[1]);
+        const int tm_e1 = from_above(tm[2]);
+        {  // U(i,j): momentum_u, else bc_solid_u
+            MomUVals<T> v;
+            v.m_c = tm[1];
+            v.m_e = tm_e0;
+            v.m_s = tm[0];
+            v.m_se = from_above(tm[0]);
+            v.m_n = tm[2];
+            v.m_ne = tm_e1;
+            v.un_c = un[1];
+            v.un_e = from_above(un[1]);
+            v.un_w = from_below(un[1]);
+            v.un_s = un[0];
+            v.un_n = un[2];
+            v.vn_s = vn[0];
+            v.vn_se = from_above(vn[0]);
+            v.vn_c = vn[1];
+            v.vn_e = from_above(vn[1]);
+            v.e1v_s = e1v[0];
+            v.e1v_se = from_above(e1v[0]);
+            v.e1v_c = e1v[1];
+            v.e1v_e = from_above(e1v[1]);
+            v.hv_s = hv[0];
+            v.sv_s = sv[0];
+            v.hv_se = from_above(hv[0]);
+            v.sv_se = from_above(sv[0]);
+            v.hv_c = hv[1];
+            v.sv_c = sv[1];
+            v.hv_e = from_above(hv[1]);
+            v.sv_e = from_above(sv[1]);
+            v.e2t_c = e2t[0];
+            v.e2t_e = from_above(e2t[0]);
+            v.e1t_c = e1t[0];
+            v.e1t_e = from_above(e1t[0]);
+            v.ht_c = ht[0];
+            v.ht_e = from_above(ht[0]);
+            v.st_c = st[0];
+            v.st_e = from_above(st[0]);
+            v.hu_c = hu[1];
+            v.su_c = su[1];
+            v.hu_s = hu[0];
+            v.su_s = su[0];
+            v.hu_n = hu[2];
+            v.su_n = su[2];
+            v.e2u_c = e2u[1];
+            v.e2u_s = e2u[0];
+            v.e2u_n = e2u[2];
+            v.e1u_c = dxu;
+            v.fcor = fcu;
+            v.area = aru;
+            v.ssha = sau;
+            T val = T(0);
+            bool wrote = in_rect(i, j, b.xs, b.xe - 1, b.ys, b.ye) &&
+                         momentum_u_val(v, p, &val);
+            if (!wrote && in_rect(i, j, b.xs - 1, b.xe, b.ys - 1, b.ye + 1) &&
+                tm[1] * tm_e0 == 0) {
+                val = T(0);
+                wrote = true;
+            }
+            if (owner && wrote) st32(o.u, c, val);
+        }
+        {  // V(i,j): momentum_v, else bc_solid_v
+            MomVVals<T> v;
+            v.m_c = tm[1];
+            v.m_e = tm_e0;
+            v.m_n = tm[2];
+            v.m_w = from_below(tm[1]);
+            v.m_nw = from_below(tm[2]);
+            v.m_ne = tm_e1;
+            v.vn_c = vn[1];
+            v.vn_n = vn[2];
+            v.vn_s = vn[0];
+            v.vn_w = from_below(vn[1]);
+            v.vn_e = from_above(vn[1]);
+            v.un_w = from_below(un[1]);
+            v.un_nw = from_below(un[2]);
+            v.un_c = un[1];
+            v.un_n = un[2];
+            v.e2u_w = from_below(e2u[1]);
+            v.e2u_nw = from_below(e2u[2]);
+            v.e2u_c = e2u[1];
+            v.e2u_n = e2u[2];
+            v.hu_w = from_below(hu[1]);
+            v.su_w = from_below(su[1]);
+            v.hu_nw = from_below(hu[2]);
+            v.su_nw = from_below(su[2]);
+            v.hu_c = hu[1];
+            v.su_c = su[1];
+            v.hu_n = hu[2];
+            v.su_n = su[2];
+            v.e1t_c = e1t[0];
+            v.e1t_n = e1t[1];
+            v.e2t_c = e2t[0];
+            v.e2t_n = e2t[1];
+            v.ht_c = ht[0];
+            v.ht_n = ht[1];
+            v.st_c = st[0];
+            v.st_n = st[1];
+            v.e1v_c = e1v[1];
+            v.e1v_w = from_below(e1v[1]);
+            v.e1v_e = from_above(e1v[1]);
+            v.hv_c = hv[1];
+            v.sv_c = sv[1];
+            v.hv_w = from_below(hv[1]);
+            v.sv_w = from_below(sv[1]);
+            v.hv_e = from_above(hv[1]);
+            v.sv_e = from_above(sv[1]);
+            v.e2v_c = dyv;
+            v.fcor = fcv;
+            v.area = arv;
+            v.ssha = sav;
+            T val = T(0);
+            bool wrote = in_rect(i, j, b.xs, b.xe, b.ys, b.ye - 1) &&
+                         momentum_v_val(v, p, &val);
+            if (!wrote && in_rect(i, j, b.xs - 1, b.xe + 1, b.ys - 1, b.ye) &&
+                tm[1] * tm[2] == 0) {
+                val = T(0);
+                wrote = true;
+            }
+            if (owner && wrote) st32(o.v, c, val);
+        }
+        {  // next_sshu(i,j), next_sshv(i,j)
+            const T at_e0 = from_above(at[0]);
+            T val;
+            if (owner && in_rect(i, j, b.xs, b.xe - 1, b.ys, b.ye) &&
+                next_ssh_pt(ssh_c, ssh_e, at[0], at_e0, aru, tm[1], tm_e0, &val))
+                st32(o.sshu, c, val);
+            if (owner && in_rect(i, j, b.xs, b.xe, b.ys, b.ye - 1) &&
+                next_ssh_pt(ssh_c, ssh_n, at[0], at[1], arv, tm[1], tm[2], &val))
+                st32(o.sshv, c, val);
+        }
+
+#define NEMO_SHIFT3(a) \
+    a[0] = a[1];       \
+    a[1] = a[2];
+#define NEMO_SHIFT2(a) a[0] = a[1];
+        NEMO_SHIFT3(un) NEMO_SHIFT3(vn) NEMO_SHIFT3(hu) NEMO_SHIFT3(su)
+        NEMO_SHIFT3(hv) NEMO_SHIFT3(sv) NEMO_SHIFT3(e1v) NEMO_SHIFT3(e2u)
+        NEMO_SHIFT2(ht) NEMO_SHIFT2(st) NEMO_SHIFT2(e1t) NEMO_SHIFT2(e2t) NEMO_SHIFT2(at)
+#undef NEMO_SHIFT3
+#undef NEMO_SHIFT2
+        tm[0] = tm[1];
+        tm[1] = tm[2];
+        tm[2] = tm[3];
+        ssh_c = ssh_n;
+    }
+}
+
+struct Ssh {
+    float hi, lo;
+};
+
+__device__ __forceinline__ Ssh split(double s)
+{
+    const float hi = (float)s;
+    return Ssh{hi, (float)(s - (double)hi)};
+}
+
+// FP32 flux divergence of T(i,j) times rdt/area (continuity_kern.c:110-116
+// without the "sshn +").
+__device__ __forceinline__ float ssh_increment(const ContVals<float> &v, float rdt)
+{
+    const float flux_e = (v.su_e + v.hu_e) * v.un_e;
+    const float flux_w = (v.su_w + v.hu_w) * v.un_w;
+    const float flux_n = (v.sv_n + v.hv_n) * v.vn_n;
+    const float flux_s = (v.sv_s + v.hv_s) * v.vn_s;
+    return divide((flux_w - flux_e + flux_s - flux_n) * rdt, v.area);
+}
+
+// One mixed-precision tile: as tile_f64 over FP32 arrays, the new ssh as a
+// compensated (hi, lo) pair accumulated in FP64.
+template <int R>
+__device__ __forceinline__ void tile_mixed(const MixedIn &f,
+                                           const StepScalars<float> &p,
+                                           const MixedOut &o, int W,
+                                           unsigned base,
+                                           const Bounds &b, int strip, int band,
+                                           Ssh tide)
+{
+    using T = float;
+    const int xmax = b.xe + 1, ymax = b.ye + 1;
+    const int lane = (int)(threadIdx.x & 63);  // lane within the wavefront
+    const int i = b.xs - 1 + strip * OUT + lane - 1;
+    const bool owner = lane >= 1 && lane <= OUT && i <= xmax;
+    const int ci = min(max(i, 0), xmax);
+    const int jlo = b.ys - 1 + band * R;
+    const int jhi = min(jlo + R, ymax + 1);
+    auto rowoff = [&](int j) { return base + (unsigned)(min(max(j, 0), ymax) * W); };
+
+    T un[3], vn[3], hu[3], su[3], hv[3], sv[3], e1v[3], e2u[3];
+    T ht[2], st[2], e1t[2], e2t[2], at[2];
+    T st_lo1;  // lo word of T(i, j+1)
+    int tm[4];
+
+    auto load3 = [&](int j, int s) {
+        const unsigned c = rowoff(j) + ci;
+        un[s] = ld32(f.un, c);
+        vn[s] = ld32(f.vn, c);
+        hu[s] = ld32(f.hu, c);
+        su[s] = ld32(f.sshn_u, c);
+        hv[s] = ld32(f.hv, c);
+        sv[s] = ld32(f.sshn_v, c);
+        e1v[s] = ld32(f.dx_v, c);
+        e2u[s] = ld32(f.dy_u, c);
+    };
+    auto load2 = [&](int j, int s) {
+        const unsigned c = rowoff(j) + ci;
+        ht[s] = ld32(f.ht, c);
+        st[s] = ld32(f.ssh_hi, c);
+        e1t[s] = ld32(f.dx_t, c);
+        e2t[s] = ld32(f.dy_t, c);
+        at[s] = ld32(f.area_t, c);
+    };
+    // new (hi, lo) ssh of T(i, r) from 3-slot s / s-1, 2-slot s2, tm t-1..t+1
+    auto new_ssh = [&](int s, int s2, int t, float lo) {
+        ContVals<T> v;
+        v.sshn = st[s2];
+        v.su_e = su[s];
+        v.hu_e = hu[s];
+        v.un_e = un[s];
+        v.su_w = from_below(su[s]);
+        v.hu_w = from_below(hu[s]);
+        v.un_w = from_below(un[s]);
+        v.sv_n = sv[s];
+        v.hv_n = hv[s];
+        v.vn_n = vn[s];
+        v.sv_s = sv[s - 1];
+        v.hv_s = hv[s - 1];
+        v.vn_s = vn[s - 1];
+        v.area = at[s2];
+        const int m_w = from_below(tm[t]), m_e = from_above(tm[t]);
+        const float inc = ssh_increment(v, p.rdt);
+        if (tide_applies(tm[t], tm[t - 1], tm[t + 1], m_e, m_w)) return tide;
+        return split((double)st[s2] + (double)lo + (double)inc);
+    };
+
+    load3(jlo - 1, 0);
+    load3(jlo, 1);
+    load2(jlo, 0);
+    tm[0] = ld32(f.tmask, rowoff(jlo - 1) + ci);
+    tm[1] = ld32(f.tmask, rowoff(jlo) + ci);
+    tm[2] = ld32(f.tmask, rowoff(jlo + 1) + ci);
+    Ssh ssh_c = new_ssh(1, 0, 1, ld32(f.ssh_lo, rowoff(jlo) + ci));
+
+    for (int j = jlo; j < jhi; j++) {
+        load3(j + 1, 2);
+        load2(j + 1, 1);
+        st_lo1 = ld32(f.ssh_lo, rowoff(j + 1) + ci);
+        tm[3] = ld32(f.tmask, rowoff(j + 2) + ci);
+        const unsigned c = rowoff(j) + ci;
+        const T dxu = ld32(f.dx_u, c), dyv = ld32(f.dy_v, c), fcu = ld32(f.fcor_u, c), fcv = ld32(f.fcor_v, c);
+        const T aru = ld32(f.area_u, c), arv = ld32(f.area_v, c), sau = ld32(f.ssha_u, c), sav = ld32(f.ssha_v, c);
+
+        const Ssh ssh_n = new_ssh(2, 1, 2, st_lo1);
+        const float ssh_e = from_above(ssh_c.hi);
+
+        if (owner && in_rect(i, j, b.xs, b.xe, b.ys, b.ye)) {
+            st32(o.ssh_hi, c, ssh_c.hi);
+            st32(o.ssh_lo, c, ssh_c.lo);
+        }
+        const int tm_e0 = from_above(tm[1]);
+        const int tm_e1 = from_above(tm[2]);
+        {
+            MomUVals<T> v;
+            v.m_c = tm[1];
+            v.m_e = tm_e0;
+            v.m_s = tm[0];
+            v.m_se = from_above(tm[0]);
+            v.m_n = tm[2];
+            v.m_ne = tm_e1;
+            v.un_c = un[1];
+            v.un_e = from_above(un[1]);
+            v.un_w = from_below(un[1]);
+            v.un_s = un[0];
+            v.un_n = un[2];
+            v.vn_s = vn[0];
+            v.vn_se = from_above(vn[0]);
+            v.vn_c = vn[1];
+            v.vn_e = from_above(vn[1]);
+            v.e1v_s = e1v[0];
+            v.e1v_se = from_above(e1v[0]);
+            v.e1v_c = e1v[1];
+            v.e1v_e = from_above(e1v[1]);
+            v.hv_s = hv[0];
+            v.sv_s = sv[0];
+            v.hv_se = from_above(hv[0]);
+            v.sv_se = from_above(sv[0]);
+            v.hv_c = hv[1];
+            v.sv_c = sv[1];
+            v.hv_e = from_above(hv[1]);
+            v.sv_e = from_above(sv[1]);
+            v.e2t_c = e2t[0];
+            v.e2t_e = from_above(e2t[0]);
+            v.e1t_c = e1t[0];
+            v.e1t_e = from_above(e1t[0]);
+            v.ht_c = ht[0];
+            v.ht_e = from_above(ht[0]);
+            v.st_c = st[0];
+            v.st_e = from_above(st[0]);
+            v.hu_c = hu[1];
+            v.su_c = su[1];
+            v.hu_s = hu[0];
+            v.su_s = su[0];
+            v.hu_n = hu[2];
+            v.su_n = su[2];
+            v.e2u_c = e2u[1];
+            v.e2u_s = e2u[0];
+            v.e2u_n = e2u[2];
+            v.e1u_c = dxu;
+            v.fcor = fcu;
+            v.area = aru;
+            v.ssha = sau;
+            T val = T(0);
+            bool wrote = in_rect(i, j, b.xs, b.xe - 1, b.ys, b.ye) &&
+                         momentum_u_val(v, p, &val);
+            if (!wrote && in_rect(i, j, b.xs - 1, b.xe, b.ys - 1, b.ye + 1) &&
+                tm[1] * tm_e0 == 0) {
+                val = T(0);
+                wrote = true;
+            }
+            if (owner && wrote) st32(o.u, c, val);
+        }
+        {
+            MomVVals<T> v;
+            v.m_c = tm[1];
+            v.m_e = tm_e0;
+            v.m_n = tm[2];
+            v.m_w = from_below(tm[1]);
+            v.m_nw = from_below(tm[2]);
+            v.m_ne = tm_e1;
+            v.vn_c = vn[1];
+            v.vn_n = vn[2];
+            v.vn_s = vn[0];
+            v.vn_w = from_below(vn[1]);
+            v.vn_e = from_above(vn[1]);
+            v.un_w = from_below(un[1]);
+            v.un_nw = from_below(un[2]);
+            v.un_c = un[1];
+            v.un_n = un[2];
+            v.e2u_w = from_below(e2u[1]);
+            v.e2u_nw = from_below(e2u[2]);
+            v.e2u_c = e2u[1];
+            v.e2u_n = e2u[2];
+            v.hu_w = from_below(hu[1]);
+            v.su_w = from_below(su[1]);
+            v.hu_nw = from_below(hu[2]);
+            v.su_nw = from_below(su[2]);
+            v.hu_c = hu[1];
+            v.su_c = su[1];
+            v.hu_n = hu[2];
+            v.su_n = su[2];
+            v.e1t_c = e1t[0];
+            v.e1t_n = e1t[1];
+            v.e2t_c = e2t[0];
+            v.e2t_n = e2t[1];
+            v.ht_c = ht[0];
+            v.ht_n = ht[1];
+            v.st_c = st[0];
+            v.st_n = st[1];
+            v.e1v_c = e1v[1];
+            v.e1v_w = from_below(e1v[1]);
+            v.e1v_e = from_above(e1v[1]);
+            v.hv_c = hv[1];
+            v.sv_c = sv[1];
+            v.hv_w = from_below(hv[1]);
+            v.sv_w = from_below(sv[1]);
+            v.hv_e = from_above(hv[1]);
+            v.sv_e = from_above(sv[1]);
+            v.e2v_c = dyv;
+            v.fcor = fcv;
+            v.area = arv;
+            v.ssha = sav;
+            T val = T(0);
+            bool wrote = in_rect(i, j, b.xs, b.xe, b.ys, b.ye - 1) &&
+                         momentum_v_val(v, p, &val);
+            if (!wrote && in_rect(i, j, b.xs - 1, b.xe + 1, b.ys - 1, b.ye) &&
+                tm[1] * tm[2] == 0) {
+                val = T(0);
+                wrote = true;
+            }
+            if (owner && wrote) st32(o.v, c, val);
+        }
+        {
+            const T at_e0 = from_above(at[0]);
+            T val;
+            if (owner && in_rect(i, j, b.xs, b.xe - 1, b.ys, b.ye) &&
+                next_ssh_pt(ssh_c.hi, ssh_e, at[0], at_e0, aru, tm[1], tm_e0, &val))
+                st32(o.sshu, c, val);
+            if (owner && in_rect(i, j, b.xs, b.xe, b.ys, b.ye - 1) &&
+                next_ssh_pt(ssh_c.hi, ssh_n.hi, at[0], at[1], arv, tm[1], tm[2], &val))
+                st32(o.sshv, c, val);
+        }
+
+#define NEMO_SHIFT3(a) \
+    a[0] = a[1];       \
+    a[1] = a[2];
+#define NEMO_SHIFT2(a) a[0] = a[1];
+        NEMO_SHIFT3(un) NEMO_SHIFT3(vn) NEMO_SHIFT3(hu) NEMO_SHIFT3(su)
+        NEMO_SHIFT3(hv) NEMO_SHIFT3(sv) NEMO_SHIFT3(e1v) NEMO_SHIFT3(e2u)
+        NEMO_SHIFT2(ht) NEMO_SHIFT2(st) NEMO_SHIFT2(e1t) NEMO_SHIFT2(e2t) NEMO_SHIFT2(at)
+#undef NEMO_SHIFT3
+#undef NEMO_SHIFT2
+        tm[0] = tm[1];
+        tm[1] = tm[2];
+        tm[2] = tm[3];
+        ssh_c = ssh_n;
+    }
+}
+
+}  // namespace wave
+}  // namespace nemo

@@ -1,0 +1,187 @@
+"""Batched ensemble (nemo_ens_*) on the GPU, through the C ABI.
+
+FP64: every domain of the ensemble is bitwise the oracle run of that domain
+alone (rows 0 .. N-2, the rows the fused form writes like the reference),
+for every launch shape. Mixed precision: every domain is bitwise the
+single-domain mixed form (nemo_mp_*), which test_gpu_mixed pins to the
+oracle within its stated tolerances."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import assert_bitwise, rel_err
+from psyclonebench_amd import _native
+from psyclonebench_amd.ensemble import EnsembleStepper, ensemble_configs
+from psyclonebench_amd.model import Model
+from psyclonebench_amd.psy import MixedStepper, TimeStepper
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("sshn_t", "un", "vn", "sshn_u", "sshn_v")
+
+
+@pytest.fixture(autouse=True)
+def _release():
+    yield
+    _native.hip_lib().nemo_hip_release_all()
+
+
+def small_ensemble(ndom=6, seed=5, nsteps=40):
+    # uneven sides 20..130 (ragged strips and bands), own depth and tide
+    return ensemble_configs(ndom=ndom, seed=seed, n_min=20, n_max=130, nsteps=nsteps)
+
+
+def oracle_domain(cfg, nsteps):
+    m = Model(cfg)
+    g = m.grid
+    arrays = m.arrays()
+    oracle.set_tide(cfg.tide_amp, cfg.tide_omega)
+    try:
+        oracle.run_steps(arrays, cfg.nit000, nsteps, g.bounds, g.W, m.scalars,
+                         kind="oracle_omp")
+    finally:
+        oracle.oracle_lib().oracle_reset_tide()
+    return {n: m.array(n) for n in FIELDS}, g
+
+
+def test_ensemble_f64_matches_oracle_per_domain():
+    cfgs = small_ensemble()
+    nsteps = 40
+    ens = EnsembleStepper([Model(c) for c in cfgs], precision=64)
+    print("ensemble points", ens.interior_points, flush=True)
+    ens.run(nsteps)
+    for d, cfg in enumerate(cfgs):
+        ref, g = oracle_domain(cfg, nsteps)
+        for n in FIELDS:
+            assert_bitwise(ens.fetch(d, n)[:g.N - 1], ref[n][:g.N - 1],
+                           f"domain {d} ({g.M}x{g.N}) {n}")
+    ens.release()
+
+
+def test_ensemble_every_shape_bitwise_equal():
+    cfgs = small_ensemble(ndom=5, seed=7)
+    models = [Model(c) for c in cfgs]
+    probe = EnsembleStepper(models, precision=64)
+    nshape = len(probe.shapes())
+    probe.release()
+    ref = None
+    for k in range(nshape):
+        ens = EnsembleStepper(models, precision=64)
+        ens.set_shape(k)
+        print("shape", k, ens.shapes()[k], flush=True)
+        ens.run(12)
+        out = [{n: ens.fetch(d, n) for n in ("sshn_t", "un", "vn")}
+               for d in range(len(cfgs))]
+        ens.release()
+        if ref is None:
+            ref = out
+            continue
+        for d in range(len(cfgs)):
+            for n in out[d]:
+                assert_bitwise(out[d][n], ref[d][n], f"shape {k} domain {d} {n}")
+
+
+def test_ensemble_graph_equals_timed_and_single_calls():
+    cfgs = small_ensemble(ndom=4, seed=9)
+    a = EnsembleStepper([Model(c) for c in cfgs])
+    b = EnsembleStepper([Model(c) for c in cfgs])
+    c = EnsembleStepper([Model(c) for c in cfgs])
+    a.run(17)
+    ms = b.timed_run(17)
+    assert ms > 0
+    for _ in range(17):
+        c.run(1)
+    for d in range(len(cfgs)):
+        for n in FIELDS:
+            x = a.fetch(d, n)
+            assert_bitwise(b.fetch(d, n), x, f"timed domain {d} {n}")
+            assert_bitwise(c.fetch(d, n), x, f"single domain {d} {n}")
+    for e in (a, b, c):
+        e.release()
+
+
+def test_ensemble_autotune_restores_state():
+    cfgs = small_ensemble(ndom=4, seed=11)
+    a = EnsembleStepper([Model(c) for c in cfgs])
+    b = EnsembleStepper([Model(c) for c in cfgs])
+    a.run(5)
+    b.run(5)
+    best, ms = a.autotune(2)
+    print("autotune:", best, a.shapes()[best], [round(x, 4) for x in ms], flush=True)
+    assert 0 <= best < len(ms) and all(x > 0 for x in ms)
+    assert ms[best] == min(ms)
+    a.run(9)
+    b.run(9)
+    for d in range(len(cfgs)):
+        for n in FIELDS:
+            assert_bitwise(a.fetch(d, n), b.fetch(d, n), f"domain {d} {n}")
+    a.release()
+    b.release()
+
+
+def test_ensemble_autotune_before_first_step():
+    cfgs = small_ensemble(ndom=3, seed=13)
+    a = EnsembleStepper([Model(c) for c in cfgs])
+    b = EnsembleStepper([Model(c) for c in cfgs])
+    a.autotune(1)
+    a.run(8)
+    b.run(8)
+    for d in range(len(cfgs)):
+        for n in FIELDS:
+            assert_bitwise(a.fetch(d, n), b.fetch(d, n), f"domain {d} {n}")
+    a.release()
+    b.release()
+
+
+def test_ensemble_mixed_matches_single_domain_mixed():
+    cfgs = small_ensemble(ndom=5, seed=17, nsteps=60)
+    ens = EnsembleStepper([Model(c) for c in cfgs], precision=32)
+    ens.run(60)
+    for d, cfg in enumerate(cfgs):
+        mp = MixedStepper(Model(cfg))
+        mp.run(60)
+        for n in FIELDS:
+            assert_bitwise(ens.fetch(d, n), mp.fetch(n), f"domain {d} {n}")
+        mp.release()
+    ens.release()
+
+
+def test_ensemble_mixed_error_against_f64():
+    cfgs = small_ensemble(ndom=4, seed=19, nsteps=400)
+    f64 = EnsembleStepper([Model(c) for c in cfgs], precision=64)
+    f32 = EnsembleStepper([Model(c) for c in cfgs], precision=32)
+    f64.run(400)
+    f32.run(400)
+    for d, cfg in enumerate(cfgs):
+        g = f64.models[d].grid
+        sl = (slice(g.ys, g.ye + 1), slice(g.xs, g.xe + 1))
+        e = {n: rel_err(f32.fetch(d, n)[sl], f64.fetch(d, n)[sl]) for n in ("sshn_t", "un", "vn")}
+        print("domain", d, e, flush=True)
+        assert e["sshn_t"][0] <= 1e-5, e
+        assert e["un"][0] <= 1e-4 and e["vn"][0] <= 1e-4, e
+    f64.release()
+    f32.release()
+
+
+@pytest.mark.slow
+def test_ensemble_256_domains_sample_equals_single_domain_fused():
+    """The 256-domain ensemble (sides 64..1024): 30 graph-replayed steps;
+    a sample of domains, smallest and largest included, is bitwise the
+    single-domain fused form of that domain."""
+    cfgs = ensemble_configs(ndom=256, seed=0, nsteps=30)
+    ens = EnsembleStepper([Model(c) for c in cfgs], precision=64)
+    print("256 domains, interior points", ens.interior_points, flush=True)
+    ens.run(30)
+    sizes = [(c.jpiglo - 2) * (c.jpjglo - 2) for c in cfgs]
+    sample = sorted({int(np.argmin(sizes)), int(np.argmax(sizes)), 0, 100, 255})
+    for d in sample:
+        st = TimeStepper(Model(cfgs[d]), mode=2)
+        st.run(30)
+        g = st.model.grid
+        for n in FIELDS:
+            assert_bitwise(ens.fetch(d, n)[:g.N - 1], st.fetch(n)[:g.N - 1],
+                           f"domain {d} {n}")
+        st.release()
+    ens.release()
