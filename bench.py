@@ -56,6 +56,10 @@ def parse_args():
     p.add_argument("--no-ensemble", action="store_true", help="skip the ensemble leg")
     p.add_argument("--ens-domains", type=int, default=256)
     p.add_argument("--ens-steps", type=int, default=2000)
+    p.add_argument("--no-closure", action="store_true", help="skip the learned-closure leg")
+    p.add_argument("--closure-seeds", default="64:66",
+                   help="held-out bathymetry seeds of the closure error check")
+    p.add_argument("--closure-steps", type=int, default=2000)
     return p.parse_args()
 
 
@@ -177,6 +181,46 @@ def mixed_leg(pristine, a, pts, f64_ms_per_step, f64_kernel_ms, f64_variant) -> 
                      "bytes_per_launch_alg": b_alg},
         "error_vs_f64": {"metric": "(relmax, relrms) over the internal T region",
                          "after_steps": errs},
+    }
+
+
+def closure_leg(a) -> dict:
+    """The learned coarse closure (trained weights shipped with the package):
+    wall time of one coarse step (1024^2, dt 80 s) plus the network's
+    correction against the fine FP64 run over the same 80 s (4 steps at
+    4096^2), the network's MFMA rate, and its error against the box-filtered
+    fine run next to the plain coarse step's, on held-out bathymetries."""
+    from psyclonebench_amd import closure_run as cr
+    from psyclonebench_amd.closure import FLOP_PER_POINT
+
+    w = cr.DEFAULT_WEIGHTS
+    if not w.exists():
+        return {"skipped": f"no trained weights at {w.relative_to(ROOT)}"}
+    sp = cr.throughput(a.n, 4, w)
+    pts = (a.n // 4) ** 2
+    flops = FLOP_PER_POINT * pts
+    achieved = flops / (sp["forward_ms"] * 1e-3) / 1e12
+    ev = cr.evaluate(a.n, 4, cr.seeds(a.closure_seeds), a.closure_steps, 500, w,
+                     log_fn=lambda *x: None)
+    return {
+        "workload": (f"coarse {a.n // 4}^2 (dx 4 km, dt 80 s) with the learned correction "
+                     f"vs fine {a.n}^2 FP64 (dx 1 km, dt 20 s); seeds {a.closure_seeds} "
+                     f"held out, {a.closure_steps} coarse steps from rest"),
+        "speedup_vs_fine": sp["speedup_vs_fine"],
+        "ms_per_coarse_step": sp["learned_ms_per_coarse_step"],
+        "plain_ms_per_coarse_step": sp["plain_ms_per_coarse_step"],
+        "fine_ms_per_80s": sp["fine_ms_per_coarse_interval"],
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": achieved / 2500.0, "kernel": "k_conv (4 layers, bf16)",
+                     "forward_ms": sp["forward_ms"], "flop_per_forward_alg": flops},
+        "error_vs_fine": {"metric": "relative RMS over the interior vs the box-filtered "
+                                    "fine state, mean over every 500 steps",
+                          "ratio_learned_over_plain_mean": ev["ratio_mean"],
+                          "ratio_learned_over_plain_final": ev["ratio_final"],
+                          "per_seed": {str(k): {"plain_mean": v["mean"]["plain"],
+                                                "learned_mean": v["mean"]["learned"],
+                                                "guard_trips": v["guard_trips"]}
+                                       for k, v in ev["seeds"].items()}},
     }
 
 
@@ -340,6 +384,8 @@ def main():
     }
     if not a.no_mixed:
         out["mixed"] = mixed_leg(pristine, a, pts, wall * 1e3 / a.steps, kernel_ms, best)
+    if not a.no_closure:
+        out["closure"] = closure_leg(a)
     if not a.no_ensemble:
         out["ensemble"] = ensemble_leg(a)
     if not a.no_cpu:

@@ -221,6 +221,80 @@ NEMO_API long long nemo_ens_interior_points(void *handle);
 NEMO_API void nemo_ens_read(void *handle, int domain, int field, double *out);
 NEMO_API void nemo_ens_destroy(void *handle);
 
+/* ---- learned coarse-grid closure (no reference counterpart) -----------
+ * A 4-layer 5x5 convolutional network (4 -> 32 -> 32 -> 32 -> 3 channels,
+ * GELU) on the coarse grid's interior T points (nx x ny, nx a multiple of
+ * 32), bf16 MFMA with FP32 accumulation; inputs u, v, ssh, depth, outputs
+ * corrections to u, v, ssh. Parameters are FP32, nemo_cl_param_count() of
+ * them, layer by layer: weights [25 taps][cout][cin] (cin 16 for layer 0 of
+ * which 4 are used, cout 16 for layer 3 of which 3 are used) then bias.
+ * Coarse-state arguments are device pointers of a coarse model created with
+ * c_invoke_time_step (the buffers its slots hold: ssh = sshn_t, u = un,
+ * v = vn, sshu = sshn_u, sshv = sshn_v, then ht, tmask, area_t/u/v), pitch
+ * coarse_width; every call is queued on the library stream. */
+NEMO_API void *nemo_cl_create(int nx, int ny, int coarse_width, int coarse_height,
+                              const float *init_params);
+NEMO_API int nemo_cl_param_count(void);
+NEMO_API void nemo_cl_layer_info(int layer, int *cin, int *cout, int *offset);
+NEMO_API void nemo_cl_get_params(void *handle, float *out);
+NEMO_API void nemo_cl_set_params(void *handle, const float *in);
+/* network input c = (x_c - in_shift[c]) / in_scale[c] for c = u, v, ssh,
+ * depth; correction k = output_k * out_scale[k] for k = u, v, ssh */
+NEMO_API void nemo_cl_set_norm(void *handle, const double *in_shift,
+                               const double *in_scale, const double *out_scale);
+/* kind 0: SGD with momentum beta1; kind 1: Adam */
+NEMO_API void nemo_cl_set_optimizer(void *handle, int kind, double lr, double beta1,
+                                    double beta2, double eps);
+/* correction kept when finite and E_after <= (1 + rel) E_before + abs_tol,
+ * E = sum over interior cells of g ssh^2 + ht (u^2 + v^2) */
+NEMO_API void nemo_cl_set_guard(void *handle, double rel, double abs_tol, double g);
+/* box filter of a fine state (device pointers, pitch fine_width) by
+ * `factor` onto the coarse grid, kept in the handle */
+NEMO_API void nemo_cl_filter(void *handle, const double *fine_ssh, const double *fine_u,
+                             const double *fine_v, int fine_width, int factor);
+NEMO_API void nemo_cl_read_filtered(void *handle, int field, double *out);
+/* coarse state := filtered fine state (wet interior cells and faces; face
+ * ssh recomputed as next_sshu/v do) */
+NEMO_API void nemo_cl_set_state(void *handle, double *ssh, double *u, double *v,
+                                double *sshu, double *sshv, const double *ht,
+                                const int *tmask, const double *area_t,
+                                const double *area_u, const double *area_v);
+/* one optimizer step on the pair (coarse state after a plain step, the
+ * filtered fine state of the same time): loss = mean squared error of the
+ * normalised correction */
+NEMO_API void nemo_cl_train_step(void *handle, double *ssh, double *u, double *v,
+                                 double *sshu, double *sshv, const double *ht,
+                                 const int *tmask, const double *area_t,
+                                 const double *area_u, const double *area_v);
+NEMO_API double nemo_cl_last_loss(void *handle);
+/* gradients scaled to a global L2 norm of at most max_norm before the
+ * optimizer step (0: no clipping) */
+NEMO_API void nemo_cl_set_clip(void *handle, double max_norm);
+/* out[7]: mean squared error of the last loss per channel (u, v, ssh),
+ * gradient norm of the last optimizer step, and the energies before / after
+ * the last guarded correction and its non-finite count */
+NEMO_API void nemo_cl_last_stats(void *handle, double *out);
+/* applies the network correction to the coarse state; with guard != 0 the
+ * energy check decides on the device whether it is kept (else the plain
+ * step stands) */
+NEMO_API void nemo_cl_correct(void *handle, int guard, double *ssh, double *u, double *v,
+                              double *sshu, double *sshv, const double *ht,
+                              const int *tmask, const double *area_t,
+                              const double *area_u, const double *area_v);
+NEMO_API int nemo_cl_guard_trips(void *handle);
+/* HIP events around the forward pass of later calls; ms of the last one */
+NEMO_API void nemo_cl_set_timing(void *handle, int on);
+NEMO_API double nemo_cl_forward_ms(void *handle);
+/* test hooks: forward / loss gradient of normalised host inputs
+ * [ny*nx][4] and targets [ny*nx][3]; outputs [ny*nx][3], gradients in the
+ * parameter layout; returns the loss */
+NEMO_API void nemo_cl_forward_host(void *handle, const float *in4, float *out3);
+NEMO_API double nemo_cl_grad_host(void *handle, const float *in4, const float *target3,
+                                  float *grads);
+/* the same followed by one optimizer step */
+NEMO_API double nemo_cl_train_host(void *handle, const float *in4, const float *target3);
+NEMO_API void nemo_cl_destroy(void *handle);
+
 /* Releases the device buffers, graphs and lists of one context (identified
  * by its ssha_t slot), or of all contexts. Slots become dangling. */
 NEMO_API void nemo_hip_release(void **ssha_t_device);

@@ -114,6 +114,37 @@ def hip_lib() -> C.CDLL:
         lib.nemo_ens_interior_points.restype = C.c_longlong
         lib.nemo_ens_read.argtypes = [vp, C.c_int, C.c_int, _dp]
         lib.nemo_ens_destroy.argtypes = [vp]
+        fp = C.POINTER(C.c_float)
+        st = [vp] * 10  # coarse state: ssh, u, v, sshu, sshv, ht, tmask, area_t/u/v
+        lib.nemo_cl_create.argtypes = [C.c_int] * 4 + [fp]
+        lib.nemo_cl_create.restype = vp
+        lib.nemo_cl_param_count.restype = C.c_int
+        lib.nemo_cl_layer_info.argtypes = [C.c_int, _ip, _ip, _ip]
+        lib.nemo_cl_get_params.argtypes = [vp, fp]
+        lib.nemo_cl_set_params.argtypes = [vp, fp]
+        lib.nemo_cl_set_norm.argtypes = [vp, _dp, _dp, _dp]
+        lib.nemo_cl_set_optimizer.argtypes = [vp, C.c_int] + [C.c_double] * 4
+        lib.nemo_cl_set_guard.argtypes = [vp] + [C.c_double] * 3
+        lib.nemo_cl_filter.argtypes = [vp, vp, vp, vp, C.c_int, C.c_int]
+        lib.nemo_cl_read_filtered.argtypes = [vp, C.c_int, _dp]
+        lib.nemo_cl_set_state.argtypes = [vp] + st
+        lib.nemo_cl_train_step.argtypes = [vp] + st
+        lib.nemo_cl_last_loss.argtypes = [vp]
+        lib.nemo_cl_last_loss.restype = C.c_double
+        lib.nemo_cl_set_clip.argtypes = [vp, C.c_double]
+        lib.nemo_cl_last_stats.argtypes = [vp, _dp]
+        lib.nemo_cl_correct.argtypes = [vp, C.c_int] + st
+        lib.nemo_cl_guard_trips.argtypes = [vp]
+        lib.nemo_cl_guard_trips.restype = C.c_int
+        lib.nemo_cl_set_timing.argtypes = [vp, C.c_int]
+        lib.nemo_cl_forward_ms.argtypes = [vp]
+        lib.nemo_cl_forward_ms.restype = C.c_double
+        lib.nemo_cl_forward_host.argtypes = [vp, fp, fp]
+        lib.nemo_cl_grad_host.argtypes = [vp, fp, fp, fp]
+        lib.nemo_cl_grad_host.restype = C.c_double
+        lib.nemo_cl_train_host.argtypes = [vp, fp, fp]
+        lib.nemo_cl_train_host.restype = C.c_double
+        lib.nemo_cl_destroy.argtypes = [vp]
         lib.nemo_hip_release.argtypes = [_vpp]
         lib.nemo_hip_device_info.argtypes = [C.c_char_p, C.c_int]
         lib.nemo_hip_device_info.restype = C.c_int

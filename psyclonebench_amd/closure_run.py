@@ -1,0 +1,374 @@
+"""Training and evaluation of the learned coarse closure on the GPU.
+
+    python -m psyclonebench_amd.closure_run train --n 4096 --bathy 0:64 --steps 500 \\
+        --out psyclonebench_amd/data/closure_weights.npz
+    python -m psyclonebench_amd.closure_run eval --n 4096 --bathy 64:80 --steps 2000 \\
+        --weights psyclonebench_amd/data/closure_weights.npz
+
+Training: for every bathymetry seed (the fine configuration is the 4096^2
+FP64 workload with that seed: depth 100 m + 50 m relief, open west edge, 1 m
+tide, dt 20 s, dx 1 km), the fine model runs `factor` steps per coarse step;
+the coarse model (dx 4 km, dt 80 s, box-filtered depth) takes one plain step
+to C_k+1 and the network takes one optimizer step on the pair (input C_k+1,
+target F_k+1 - C_k+1, F the box-filtered fine state). The coarse model then
+continues from its own corrected state (as at inference, energy guard on),
+and is reset to F_k+1 every r steps, r growing from 1 to --rollout over the
+run, so that the network learns to pull drifted states back toward the fine
+solution. Normalisation scales come from a pilot run of the first seed.
+
+Evaluation: for every held-out seed, the fine run, the plain coarse run and
+the coarse run with the learned correction (and the energy guard) advance
+side by side from rest; every `--every` coarse steps the coarse states are
+compared with the box-filtered fine state (relative RMS error over the
+interior, per field).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+from .closure import ClosureNet, coarse_model, init_params
+from .model import Model, c2_config
+from .psy import TimeStepper
+
+FIELDS = ("sshn_t", "un", "vn")
+# error fields: ssh over the whole interior and over the cells the step does
+# not clamp to the tide (neither coarse form can change the clamped ones)
+EVAL_FIELDS = FIELDS + ("sshn_t_free",)
+DEFAULT_WEIGHTS = Path(__file__).resolve().parent / "data" / "closure_weights.npz"
+
+
+def log(*a):
+    print(*a, flush=True)
+
+
+def seeds(spec: str) -> list[int]:
+    a, b = spec.split(":")
+    return list(range(int(a), int(b)))
+
+
+def pair(n: int, factor: int, seed: int, coarse_steps: int):
+    fine = Model(c2_config(n, factor * coarse_steps, seed=seed))
+    coarse = coarse_model(fine, factor)
+    return fine, coarse
+
+
+def interior(a: np.ndarray, g) -> np.ndarray:
+    return a[g.ys:g.ye + 1, g.xs:g.xe + 1]
+
+
+def rel_rms(x: np.ndarray, ref: np.ndarray) -> float:
+    return float(np.linalg.norm(x - ref) / max(np.linalg.norm(ref), 1e-300))
+
+
+def guard_abs(coarse: Model) -> float:
+    """Energy allowance of the guard per step: g a^2 times the cells of one
+    grid column, the energy of one column of cells raised to the tidal
+    amplitude a (about 40 times what the tide brings in per coarse step)."""
+    g = coarse.grid
+    return 9.80665 * coarse.cfg.tide_amp ** 2 * (g.ye - g.ys + 1)
+
+
+def free_cells(coarse: Model) -> np.ndarray:
+    """Interior T cells whose ssh the step does not clamp to the tide (wet,
+    no open 4-neighbour)."""
+    tm = coarse.array("tmask")
+    return interior((tm > 0) & (np.roll(tm, 1, 1) >= 0) & (np.roll(tm, -1, 1) >= 0)
+                    & (np.roll(tm, 1, 0) >= 0) & (np.roll(tm, -1, 0) >= 0), coarse.grid)
+
+
+def _active_rms(a: np.ndarray) -> float:
+    m = np.abs(a) > 1e-6 * max(float(np.max(np.abs(a))), 1e-300)
+    return float(np.sqrt(np.mean(a[m] ** 2))) if np.any(m) else 1.0
+
+
+def pilot_norm(n: int, factor: int, seed: int, steps: int, every: int = 50) -> dict:
+    """Input and target scales from a plain run of one seed (each coarse step
+    started from the filtered fine state): RMS over the cells where the field
+    is not (numerically) zero, sampled every `every` coarse steps."""
+    fine, coarse = pair(n, factor, seed, steps)
+    sf, sc = TimeStepper(fine, mode=2), TimeStepper(coarse, mode=2)
+    net = ClosureNet(coarse)
+    g = coarse.grid
+    xs = {k: [] for k in FIELDS}
+    ts = {k: [] for k in FIELDS}
+    for k in range(steps):
+        sf.run(factor)
+        sc.run(1)
+        net.filter_fine(sf, factor)
+        if (k + 1) % every == 0:
+            for f in FIELDS:
+                c = interior(sc.fetch(f), g)
+                F = interior(net.filtered(f, (g.H, g.W)), g)
+                xs[f].append(c)
+                ts[f].append(F - c)
+        net.set_state(sc)
+    net.release()
+    sf.release()
+    sc.release()
+    # the ssh of cells the step clamps to the tide is neither corrected nor
+    # learned: leave it out of the ssh scales
+    free = free_cells(coarse)
+    xs["sshn_t"] = [a[free] for a in xs["sshn_t"]]
+    ts["sshn_t"] = [a[free] for a in ts["sshn_t"]]
+    ht = interior(coarse.array("ht"), g)
+    order = ("un", "vn", "sshn_t")
+    return {"in_shift": [0.0, 0.0, 0.0, float(np.mean(ht))],
+            "in_scale": [_active_rms(np.concatenate([a.ravel() for a in xs[f]])) for f in order]
+            + [max(float(np.std(ht)), 1.0)],
+            "out_scale": [_active_rms(np.concatenate([a.ravel() for a in ts[f]])) for f in order]}
+
+
+def train(a) -> None:
+    bathy = seeds(a.bathy)
+    t0 = time.perf_counter()
+    if a.resume:
+        z = np.load(a.resume)
+        params = z["params"]
+        norm = json.loads(str(z["meta"]))["norm"]
+        log("resumed from", a.resume)
+    else:
+        params = init_params(a.seed)
+        log("pilot run for the normalisation, seed", bathy[0])
+        norm = pilot_norm(a.n, a.factor, bathy[0], a.pilot_steps)
+    log("norm", json.dumps(norm))
+    net = None
+    total = len(bathy) * a.steps
+    done = 0
+    history = []
+    spin_rng = np.random.default_rng(a.seed + 1)
+    for b in bathy:
+        fine, coarse = pair(a.n, a.factor, b, a.steps)
+        sf, sc = TimeStepper(fine, mode=2), TimeStepper(coarse, mode=2)
+        # the training window of this seed starts after a random spin-up of
+        # the fine run (0 .. --max-spin coarse intervals), so that the pairs
+        # cover the tidal cycle, not only its first hours
+        spin = int(spin_rng.integers(0, a.max_spin + 1)) if a.max_spin > 0 else 0
+        if spin:
+            sf.run(a.factor * spin)
+        sc.run(1)                 # first call: device state and pairing
+        sc.istp = coarse.cfg.nit000 + spin
+        if net is None:
+            net = ClosureNet(coarse, params=params)
+            net.set_norm(norm["in_shift"], norm["in_scale"], norm["out_scale"])
+            net.set_clip(a.clip)
+        g = coarse.grid
+        net.set_guard(a.guard_rel, guard_abs(coarse))
+        net.filter_fine(sf, a.factor)
+        net.set_state(sc)         # the window starts from the filtered fine state
+        losses = []
+        since_reset = 0
+        for k in range(a.steps):
+            frac = done / max(total - 1, 1)
+            lr = a.lr_min + 0.5 * (a.lr - a.lr_min) * (1.0 + np.cos(np.pi * frac))
+            net.set_optimizer("adam", lr=lr)
+            sf.run(a.factor)
+            sc.run(1)
+            net.filter_fine(sf, a.factor)
+            net.train_step(sc)
+            rollout = 1 + int(round((a.rollout - 1) * frac))
+            since_reset += 1
+            if since_reset >= rollout:
+                net.set_state(sc)
+                since_reset = 0
+            else:
+                net.correct(sc, guard=True)
+            done += 1
+            if (k + 1) % a.log_every == 0:
+                st = net.last_stats()
+                losses.append([st["loss_u"], st["loss_v"], st["loss_ssh"], st["grad_norm"]])
+        history.append({"seed": b, "loss_u_v_ssh_gradnorm": losses})
+        first = losses[0] if losses else [float("nan")] * 4
+        last = losses[-1] if losses else [float("nan")] * 4
+        log(f"seed {b}: loss u/v/ssh {first[0]:.3e} {first[1]:.3e} {first[2]:.3e} -> "
+            f"{last[0]:.3e} {last[1]:.3e} {last[2]:.3e}  |g| {last[3]:.2e}  lr {lr:.2e}  "
+            f"{time.perf_counter() - t0:.0f} s")
+        sf.release()
+        sc.release()
+        params = net.params()
+        save(a.out, params, norm, a, history)
+    net.release()
+    log(f"trained {done} steps in {time.perf_counter() - t0:.0f} s -> {a.out}")
+
+
+def save(path, params, norm, a, history) -> None:
+    meta = {"norm": norm, "n_fine": a.n, "factor": a.factor, "bathy": a.bathy,
+            "coarse_steps_per_seed": a.steps, "lr": a.lr, "lr_min": a.lr_min,
+            "rollout": a.rollout, "guard_rel": a.guard_rel, "clip": a.clip,
+            "max_spin": a.max_spin,
+            "optimizer": "adam", "history": history}
+    Path(path).parent.mkdir(parents=True, exist_ok=True)
+    np.savez(path, params=np.asarray(params, np.float32), meta=json.dumps(meta))
+
+
+def load_weights(path) -> tuple[np.ndarray, dict]:
+    z = np.load(path)
+    return z["params"], json.loads(str(z["meta"]))
+
+
+def evaluate(n: int, factor: int, bathy: list[int], steps: int, every: int, weights,
+             guard_rel: float = 0.05, log_fn=log, trace_energy: bool = False) -> dict:
+    params, meta = load_weights(weights)
+    norm = meta["norm"]
+    out = {"seeds": {}, "every": every, "steps": steps}
+    for b in bathy:
+        fine, coarse = pair(n, factor, b, steps)
+        plain = coarse.copy()
+        sf, sp, sl = (TimeStepper(fine, mode=2), TimeStepper(plain, mode=2),
+                      TimeStepper(coarse, mode=2))
+        net = ClosureNet(coarse, params=params)
+        net.set_norm(norm["in_shift"], norm["in_scale"], norm["out_scale"])
+        g = coarse.grid
+        net.set_guard(guard_rel, guard_abs(coarse))
+        free = free_cells(coarse)
+        errs = {"plain": {f: [] for f in EVAL_FIELDS}, "learned": {f: [] for f in EVAL_FIELDS}}
+        ratios = []
+        for k in range(steps):
+            sf.run(factor)
+            sp.run(1)
+            sl.run(1)
+            net.correct(sl, guard=True)
+            if trace_energy:
+                st = net.last_stats()
+                ratios.append((st["energy_after"] - st["energy_before"])
+                              / max(st["energy_before"], 1e-300))
+            if (k + 1) % every == 0:
+                net.filter_fine(sf, factor)
+                for f in FIELDS:
+                    F = interior(net.filtered(f, (g.H, g.W)), g)
+                    for m, st in (("plain", sp), ("learned", sl)):
+                        x = interior(st.fetch(f), g)
+                        errs[m][f].append(rel_rms(x, F))
+                        if f == "sshn_t":
+                            errs[m]["sshn_t_free"].append(rel_rms(x[free], F[free]))
+        trips = net.guard_trips()
+        res = {"final": {m: {f: errs[m][f][-1] for f in EVAL_FIELDS} for m in errs},
+               "mean": {m: {f: float(np.mean(errs[m][f])) for f in EVAL_FIELDS} for m in errs},
+               "guard_trips": trips}
+        res["ratio_final"] = {f: res["final"]["learned"][f] / res["final"]["plain"][f]
+                              for f in EVAL_FIELDS}
+        res["ratio_mean"] = {f: res["mean"]["learned"][f] / res["mean"]["plain"][f]
+                             for f in EVAL_FIELDS}
+        out["seeds"][b] = res
+        log_fn(f"seed {b}: plain {res['final']['plain']}  learned {res['final']['learned']}"
+               f"  ratio(final) {res['ratio_final']}  trips {trips}")
+        if trace_energy:
+            r = np.array(ratios)
+            res["energy_change_percentiles"] = {
+                str(q): float(np.percentile(r, q)) for q in (1, 10, 50, 90, 99, 100)}
+            log_fn(f"seed {b}: relative energy change per correction, percentiles "
+                   f"{res['energy_change_percentiles']}")
+        net.release()
+        for s in (sf, sp, sl):
+            s.release()
+    for key in ("ratio_final", "ratio_mean"):
+        out[key] = {f: float(np.mean([out["seeds"][b][key][f] for b in bathy]))
+                    for f in EVAL_FIELDS}
+    return out
+
+
+def throughput(n: int, factor: int, weights, steps: int = 200) -> dict:
+    """Wall time per simulated coarse interval (factor fine steps) of the
+    fine FP64 run and of one coarse step plus the learned correction."""
+    from ._native import hip_lib
+    params, meta = load_weights(weights)
+    norm = meta["norm"]
+    fine, coarse = pair(n, factor, 0, steps)
+    sf, sl = TimeStepper(fine, mode=2), TimeStepper(coarse, mode=2)
+    sf.run(factor * 10)
+    sl.run(1)
+    net = ClosureNet(coarse, params=params)
+    net.set_norm(norm["in_shift"], norm["in_scale"], norm["out_scale"])
+    hip_lib().nemo_hip_synchronize()
+    t0 = time.perf_counter()
+    sf.run(factor * steps)
+    hip_lib().nemo_hip_synchronize()
+    fine_ms = (time.perf_counter() - t0) * 1e3 / steps
+    for _ in range(10):
+        sl.run(1)
+        net.correct(sl, guard=True)
+    hip_lib().nemo_hip_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sl.run(1)
+        net.correct(sl, guard=True)
+    hip_lib().nemo_hip_synchronize()
+    learned_ms = (time.perf_counter() - t0) * 1e3 / steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sl.run(1)
+    hip_lib().nemo_hip_synchronize()
+    plain_ms = (time.perf_counter() - t0) * 1e3 / steps
+    net.set_timing(True)
+    fwd = []
+    for _ in range(20):
+        net.correct(sl, guard=True)
+        fwd.append(net.forward_ms())
+    net.set_timing(False)
+    net.release()
+    sf.release()
+    sl.release()
+    return {"fine_ms_per_coarse_interval": fine_ms, "learned_ms_per_coarse_step": learned_ms,
+            "plain_ms_per_coarse_step": plain_ms, "forward_ms": float(np.median(fwd)),
+            "speedup_vs_fine": fine_ms / learned_ms}
+
+
+def main(argv=None) -> None:
+    p = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    sub = p.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("train")
+    t.add_argument("--n", type=int, default=4096)
+    t.add_argument("--factor", type=int, default=4)
+    t.add_argument("--bathy", default="0:64")
+    t.add_argument("--steps", type=int, default=500)
+    t.add_argument("--lr", type=float, default=2e-3)
+    t.add_argument("--lr-min", type=float, default=1e-4)
+    t.add_argument("--seed", type=int, default=0)
+    t.add_argument("--log-every", type=int, default=50)
+    t.add_argument("--rollout", type=int, default=20,
+                   help="longest stretch of steps on the corrected state between resets")
+    t.add_argument("--guard-rel", type=float, default=0.05)
+    t.add_argument("--clip", type=float, default=0.0,
+                   help="largest global gradient norm (0: no clipping)")
+    t.add_argument("--pilot-steps", type=int, default=2000,
+                   help="coarse steps of the normalisation pilot run")
+    t.add_argument("--max-spin", type=int, default=1500,
+                   help="longest spin-up of a seed's fine run before its window, in "
+                        "coarse steps")
+    t.add_argument("--resume", default=None)
+    t.add_argument("--out", default=str(DEFAULT_WEIGHTS))
+    e = sub.add_parser("eval")
+    e.add_argument("--n", type=int, default=4096)
+    e.add_argument("--factor", type=int, default=4)
+    e.add_argument("--bathy", default="64:80")
+    e.add_argument("--steps", type=int, default=2000)
+    e.add_argument("--every", type=int, default=100)
+    e.add_argument("--weights", default=str(DEFAULT_WEIGHTS))
+    e.add_argument("--json", default=None)
+    e.add_argument("--guard-rel", type=float, default=0.05)
+    e.add_argument("--trace-energy", action="store_true",
+                   help="record the energy change of every correction (synchronises)")
+    s = sub.add_parser("speed")
+    s.add_argument("--n", type=int, default=4096)
+    s.add_argument("--factor", type=int, default=4)
+    s.add_argument("--weights", default=str(DEFAULT_WEIGHTS))
+    a = p.parse_args(argv)
+    if a.cmd == "train":
+        train(a)
+    elif a.cmd == "eval":
+        res = evaluate(a.n, a.factor, seeds(a.bathy), a.steps, a.every, a.weights,
+                       guard_rel=a.guard_rel, trace_energy=a.trace_energy)
+        log(json.dumps({k: res[k] for k in ("ratio_final", "ratio_mean")}))
+        if a.json:
+            Path(a.json).write_text(json.dumps(res, indent=1))
+    else:
+        log(json.dumps(throughput(a.n, a.factor, a.weights)))
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])
