@@ -242,6 +242,10 @@ NEMO_API void nemo_cl_set_params(void *handle, const float *in);
  * depth; correction k = output_k * out_scale[k] for k = u, v, ssh */
 NEMO_API void nemo_cl_set_norm(void *handle, const double *in_shift,
                                const double *in_scale, const double *out_scale);
+/* activity gate: the correction of a point is multiplied by
+ * 1 - exp(-(a/eps)^2), a = |u| + |v| + |ssh| in normalised input units, so
+ * points at rest get none (eps <= 0: no gate; default 0) */
+NEMO_API void nemo_cl_set_gate(void *handle, double eps);
 /* kind 0: SGD with momentum beta1; kind 1: Adam */
 NEMO_API void nemo_cl_set_optimizer(void *handle, int kind, double lr, double beta1,
                                     double beta2, double eps);

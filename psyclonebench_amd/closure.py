@@ -164,6 +164,9 @@ class ClosureNet:
     def last_loss(self) -> float:
         return hip_lib().nemo_cl_last_loss(self.handle)
 
+    def set_gate(self, eps: float) -> None:
+        hip_lib().nemo_cl_set_gate(self.handle, eps)
+
     def set_clip(self, max_norm: float) -> None:
         hip_lib().nemo_cl_set_clip(self.handle, max_norm)
 

@@ -74,6 +74,12 @@ def guard_abs(coarse: Model) -> float:
     return 9.80665 * coarse.cfg.tide_amp ** 2 * (g.ye - g.ys + 1)
 
 
+def configure(net: ClosureNet, norm: dict) -> None:
+    """Normalisation and activity gate of a trained network."""
+    net.set_norm(norm["in_shift"], norm["in_scale"], norm["out_scale"])
+    net.set_gate(norm.get("gate_eps", 0.0))
+
+
 def free_cells(coarse: Model) -> np.ndarray:
     """Interior T cells whose ssh the step does not clamp to the tide (wet,
     no open 4-neighbour)."""
@@ -136,6 +142,8 @@ def train(a) -> None:
         params = init_params(a.seed)
         log("pilot run for the normalisation, seed", bathy[0])
         norm = pilot_norm(a.n, a.factor, bathy[0], a.pilot_steps)
+    if a.gate_eps is not None:
+        norm["gate_eps"] = a.gate_eps
     log("norm", json.dumps(norm))
     net = None
     total = len(bathy) * a.steps
@@ -155,7 +163,7 @@ def train(a) -> None:
         sc.istp = coarse.cfg.nit000 + spin
         if net is None:
             net = ClosureNet(coarse, params=params)
-            net.set_norm(norm["in_shift"], norm["in_scale"], norm["out_scale"])
+            configure(net, norm)
             net.set_clip(a.clip)
         g = coarse.grid
         net.set_guard(a.guard_rel, guard_abs(coarse))
@@ -222,7 +230,7 @@ def evaluate(n: int, factor: int, bathy: list[int], steps: int, every: int, weig
         sf, sp, sl = (TimeStepper(fine, mode=2), TimeStepper(plain, mode=2),
                       TimeStepper(coarse, mode=2))
         net = ClosureNet(coarse, params=params)
-        net.set_norm(norm["in_shift"], norm["in_scale"], norm["out_scale"])
+        configure(net, norm)
         g = coarse.grid
         net.set_guard(guard_rel, guard_abs(coarse))
         free = free_cells(coarse)
@@ -283,7 +291,7 @@ def throughput(n: int, factor: int, weights, steps: int = 200) -> dict:
     sf.run(factor * 10)
     sl.run(1)
     net = ClosureNet(coarse, params=params)
-    net.set_norm(norm["in_shift"], norm["in_scale"], norm["out_scale"])
+    configure(net, norm)
     hip_lib().nemo_hip_synchronize()
     t0 = time.perf_counter()
     sf.run(factor * steps)
@@ -335,6 +343,8 @@ def main(argv=None) -> None:
     t.add_argument("--guard-rel", type=float, default=0.05)
     t.add_argument("--clip", type=float, default=0.0,
                    help="largest global gradient norm (0: no clipping)")
+    t.add_argument("--gate-eps", type=float, default=0.01,
+                   help="activity gate width in normalised input units (0: no gate)")
     t.add_argument("--pilot-steps", type=int, default=2000,
                    help="coarse steps of the normalisation pilot run")
     t.add_argument("--max-spin", type=int, default=1500,

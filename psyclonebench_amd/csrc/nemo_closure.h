@@ -41,6 +41,10 @@ struct CoarseState {
 struct Norm {
     float in_scale[4], in_shift[4];  // input channel c: (x - shift) / scale
     float out_scale[3];              // correction = network output * scale
+    // Activity gate: the network output of a point is multiplied by
+    // 1 - exp(-(a / gate_eps)^2), a = |u| + |v| + |ssh| in input units, so
+    // that a point at rest gets no correction (gate_eps <= 0: no gate).
+    float gate_eps;
 };
 
 // ---- network kernels -------------------------------------------------------
@@ -55,6 +59,7 @@ struct ConvArgs {
     __bf16 *pre;          // padded NHWC pre-activation (EPI_HIDDEN), may be null
     const __bf16 *gpre;   // padded NHWC pre-activation multiplying the gradient (EPI_DX)
     float *outf;          // [ny*nx][4] FP32 network output (EPI_OUTPUT)
+    const float *gate;    // [ny*nx] multiplier of the output (EPI_OUTPUT), may be null
     int nx, ny;
 };
 
@@ -80,7 +85,7 @@ void launch_weights_bf16(hipStream_t s, const float *w, int cin, int cout,
                          __bf16 *fwd, __bf16 *flip);
 // coarse state -> padded NHWC input tensor (CIN0 channels)
 void launch_pack(hipStream_t s, const CoarseState &c, const Norm &nm, __bf16 *in0,
-                 int nx, int ny);
+                 float *gate, int nx, int ny);
 // target[(y*nx+x)*4 + k] = (F_k - C_k) / out_scale_k, k = u, v, ssh
 void launch_target(hipStream_t s, const double *fssh, const double *fu, const double *fv,
                    const CoarseState &c, const Norm &nm, float *target, int nx, int ny);
@@ -89,7 +94,8 @@ void launch_target(hipStream_t s, const double *fssh, const double *fu, const do
 // mask (pitch W), the ssh of cells the step clamps to the tide is left out.
 int loss_blocks();
 void launch_loss(hipStream_t s, const float *pred, const float *target, __bf16 *dpre4,
-                 float *partial, int nx, int ny, const int *tmask, int W);
+                 float *partial, int nx, int ny, const int *tmask, int W,
+                 const float *gate);
 
 // ---- fine <-> coarse plumbing ----------------------------------------------
 // Box filter of the fine state onto the coarse grid (factor f): T cells by

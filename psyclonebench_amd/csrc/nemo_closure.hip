@@ -119,7 +119,17 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a)
         r1.y = hi[1] + a.bias[1];
         r1.z = hi[2] + a.bias[2];
         r1.w = 0.0f;
-        float4 *dst = reinterpret_cast<float4 *>(a.outf) + (size_t)y * nx + x0 + (lane & 15);
+        const size_t p = (size_t)y * nx + x0 + (lane & 15);
+        if (a.gate) {
+            const float g0 = a.gate[p], g1 = a.gate[p + 16];
+            r0.x *= g0;
+            r0.y *= g0;
+            r0.z *= g0;
+            r1.x *= g1;
+            r1.y *= g1;
+            r1.z *= g1;
+        }
+        float4 *dst = reinterpret_cast<float4 *>(a.outf) + p;
         dst[0] = r0;
         dst[16] = r1;
     };
@@ -372,17 +382,25 @@ __global__ void k_weights_bf16(const float *__restrict__ w, int cin, int cout,
     if (flip) flip[((size_t)(TAPS - 1 - tap) * cin + ci) * cout + co] = b;
 }
 
-__global__ void k_pack(CoarseState c, Norm nm, __bf16 *__restrict__ in0, int nx, int ny)
+__global__ void k_pack(CoarseState c, Norm nm, __bf16 *__restrict__ in0,
+                       float *__restrict__ gate, int nx, int ny)
 {
     const int p = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (p >= nx * ny) return;
     const int y = p / nx, x = p - y * nx;
     const size_t k = (size_t)(y + 1) * c.W + (x + 1);
     bf16x8 lo = {}, hi = {};
-    lo[0] = (__bf16)(((float)c.u[k] - nm.in_shift[0]) / nm.in_scale[0]);
-    lo[1] = (__bf16)(((float)c.v[k] - nm.in_shift[1]) / nm.in_scale[1]);
-    lo[2] = (__bf16)(((float)c.ssh[k] - nm.in_shift[2]) / nm.in_scale[2]);
+    const float u = ((float)c.u[k] - nm.in_shift[0]) / nm.in_scale[0];
+    const float v = ((float)c.v[k] - nm.in_shift[1]) / nm.in_scale[1];
+    const float s = ((float)c.ssh[k] - nm.in_shift[2]) / nm.in_scale[2];
+    lo[0] = (__bf16)u;
+    lo[1] = (__bf16)v;
+    lo[2] = (__bf16)s;
     lo[3] = (__bf16)(((float)c.ht[k] - nm.in_shift[3]) / nm.in_scale[3]);
+    if (gate) {
+        const float act = (fabsf(u) + fabsf(v) + fabsf(s)) / nm.gate_eps;
+        gate[p] = nm.gate_eps > 0.0f ? 1.0f - expf(-act * act) : 1.0f;
+    }
     bf16x8 *dst = reinterpret_cast<bf16x8 *>(in0 + pad_index(y, x, nx) * CIN0);
     dst[0] = lo;
     dst[1] = hi;
@@ -416,7 +434,8 @@ __global__ __launch_bounds__(256) void k_loss(const float *__restrict__ pred,
                                               const float *__restrict__ target,
                                               __bf16 *__restrict__ dpre4,
                                               float *__restrict__ partial, int nx, int ny,
-                                              const int *__restrict__ tmask, int W)
+                                              const int *__restrict__ tmask, int W,
+                                              const float *__restrict__ gate)
 {
     __shared__ float red[3][256];
     const int npx = nx * ny;
@@ -433,10 +452,12 @@ __global__ __launch_bounds__(256) void k_loss(const float *__restrict__ pred,
         s0 += d0 * d0;
         s1 += d1 * d1;
         s2 += d2 * d2;
+        // the output is gated (pred = gate * raw output): d loss / d raw
+        const float gs = gate ? scale * gate[p] : scale;
         bf16x8 lo = {}, hi = {};
-        lo[0] = (__bf16)(scale * d0);
-        lo[1] = (__bf16)(scale * d1);
-        lo[2] = (__bf16)(scale * d2);
+        lo[0] = (__bf16)(gs * d0);
+        lo[1] = (__bf16)(gs * d1);
+        lo[2] = (__bf16)(gs * d2);
         bf16x8 *dst = reinterpret_cast<bf16x8 *>(dpre4 + pad_index(y, x, nx) * COUT4);
         dst[0] = lo;
         dst[1] = hi;
@@ -674,9 +695,9 @@ void launch_weights_bf16(hipStream_t s, const float *w, int cin, int cout, __bf1
 }
 
 void launch_pack(hipStream_t s, const CoarseState &c, const Norm &nm, __bf16 *in0,
-                 int nx, int ny)
+                 float *gate, int nx, int ny)
 {
-    k_pack<<<grid_for((long)nx * ny), dim3(256), 0, s>>>(c, nm, in0, nx, ny);
+    k_pack<<<grid_for((long)nx * ny), dim3(256), 0, s>>>(c, nm, in0, gate, nx, ny);
     NEMO_LAUNCH_CHECK();
 }
 
@@ -691,10 +712,12 @@ void launch_target(hipStream_t s, const double *fssh, const double *fu, const do
 int loss_blocks() { return 256; }
 
 void launch_loss(hipStream_t s, const float *pred, const float *target, __bf16 *dpre4,
-                 float *partial, int nx, int ny, const int *tmask, int W)
+                 float *partial, int nx, int ny, const int *tmask, int W,
+                 const float *gate)
 {
     k_loss<<<dim3((unsigned)loss_blocks()), dim3(256), 0, s>>>(pred, target, dpre4,
-                                                               partial, nx, ny, tmask, W);
+                                                               partial, nx, ny, tmask, W,
+                                                               gate);
     NEMO_LAUNCH_CHECK();
 }
 

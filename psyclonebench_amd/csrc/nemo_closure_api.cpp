@@ -26,7 +26,7 @@ struct ClosureContext {
     float *params = nullptr, *grads = nullptr, *mom1 = nullptr, *mom2 = nullptr;
     __bf16 *wf[NLAYER] = {}, *wx[NLAYER] = {};
     __bf16 *in0 = nullptr, *act[NLAYER - 1] = {}, *pre[NLAYER - 1] = {}, *dpre[NLAYER] = {};
-    float *pred = nullptr, *target = nullptr;
+    float *pred = nullptr, *target = nullptr, *gate = nullptr;
     float *dw_partial = nullptr, *b_partial = nullptr, *loss_partial = nullptr;
     double *fssh = nullptr, *fu = nullptr, *fv = nullptr;
     double *guard_partial = nullptr;
@@ -46,7 +46,8 @@ struct ClosureContext {
     ~ClosureContext()
     {
         void *all[] = {params, grads, mom1, mom2, in0, pred, target, dw_partial, b_partial,
-                       loss_partial, fssh, fu, fv, guard_partial, flag, trips, norm_scale};
+                       loss_partial, fssh, fu, fv, guard_partial, flag, trips, norm_scale,
+                       gate};
         for (void *p : all)
             if (p) (void)hipFree(p);
         for (int l = 0; l < NLAYER; l++) {
@@ -91,8 +92,9 @@ CoarseState state(ClosureContext &C, double *ssh, double *u, double *v, double *
     return CoarseState{ssh, u, v, sshu, sshv, ht, tmask, area_t, area_u, area_v, C.Wc};
 }
 
-// Forward pass from C.in0; training keeps the pre-activations.
-void forward(ClosureContext &C, hipStream_t s, bool training)
+// Forward pass from C.in0; training keeps the pre-activations; gated: the
+// output is multiplied by the activity gate k_pack computed with the input.
+void forward(ClosureContext &C, hipStream_t s, bool training, bool gated)
 {
     if (C.timing) NEMO_HIP_CHECK(hipEventRecord(C.ev0, s));
     const __bf16 *x = C.in0;
@@ -110,6 +112,7 @@ void forward(ClosureContext &C, hipStream_t s, bool training)
             x = C.act[l];
         } else {
             a.outf = C.pred;
+            a.gate = gated ? C.gate : nullptr;
             launch_conv(s, layer_cin(l), layer_cout(l), EPI_OUTPUT, a);
         }
     }
@@ -224,6 +227,7 @@ void *nemo_cl_create(int nx, int ny, int coarse_width, int coarse_height,
     }
     const size_t npx = (size_t)nx * ny;
     C->pred = zeros<float>(npx * 4, s);
+    C->gate = zeros<float>(npx, s);
     C->target = zeros<float>(npx * 4, s);
     C->dw_partial = zeros<float>((size_t)dw_blocks() * TAPS * HID * HID, s);
     C->b_partial = zeros<float>((size_t)bias_blocks() * HID, s);
@@ -294,6 +298,8 @@ void nemo_cl_set_optimizer(void *h, int kind, double lr, double beta1, double be
     C.eps = (float)eps;
 }
 
+void nemo_cl_set_gate(void *h, double eps) { ctx(h).nm.gate_eps = (float)eps; }
+
 void nemo_cl_set_guard(void *h, double rel, double abs_tol, double g)
 {
     ClosureContext &C = ctx(h);
@@ -338,11 +344,11 @@ void nemo_cl_train_step(void *h, double *ssh, double *u, double *v, double *sshu
     ClosureContext &C = ctx(h);
     const CoarseState c = state(C, ssh, u, v, sshu, sshv, ht, tmask, area_t, area_u, area_v);
     hipStream_t s = lib_stream();
-    launch_pack(s, c, C.nm, C.in0, C.nx, C.ny);
-    forward(C, s, true);
+    launch_pack(s, c, C.nm, C.in0, C.gate, C.nx, C.ny);
+    forward(C, s, true, true);
     launch_target(s, C.fssh, C.fu, C.fv, c, C.nm, C.target, C.nx, C.ny);
     launch_loss(s, C.pred, C.target, C.dpre[NLAYER - 1], C.loss_partial, C.nx, C.ny, tmask,
-                C.Wc);
+                C.Wc, C.gate);
     backward(C, s);
     optimizer_step(C, s);
 }
@@ -394,8 +400,8 @@ void nemo_cl_correct(void *h, int guard, double *ssh, double *u, double *v, doub
     ClosureContext &C = ctx(h);
     const CoarseState c = state(C, ssh, u, v, sshu, sshv, ht, tmask, area_t, area_u, area_v);
     hipStream_t s = lib_stream();
-    launch_pack(s, c, C.nm, C.in0, C.nx, C.ny);
-    forward(C, s, false);
+    launch_pack(s, c, C.nm, C.in0, C.gate, C.nx, C.ny);
+    forward(C, s, false, true);
     if (guard) {
         launch_guard_partial(s, C.pred, C.nm, c, C.g, C.guard_partial, C.nx, C.ny);
         launch_guard_decide(s, C.guard_partial, guard_blocks(), C.guard_rel, C.guard_abs,
@@ -431,7 +437,7 @@ void nemo_cl_forward_host(void *h, const float *in4, float *out3)
     ClosureContext &C = ctx(h);
     hipStream_t s = lib_stream();
     upload_inputs(C, in4, s);
-    forward(C, s, false);
+    forward(C, s, false, false);
     const size_t npx = (size_t)C.nx * C.ny;
     std::vector<float> p(npx * 4);
     NEMO_HIP_CHECK(hipStreamSynchronize(s));
@@ -450,9 +456,9 @@ static void host_pair(ClosureContext &C, const float *in4, const float *target3,
         for (int c = 0; c < 3; c++) t[k * 4 + c] = target3[k * 3 + c];
     NEMO_HIP_CHECK(hipMemcpyAsync(C.target, t.data(), t.size() * sizeof(float),
                                   hipMemcpyHostToDevice, s));
-    forward(C, s, true);
+    forward(C, s, true, false);
     launch_loss(s, C.pred, C.target, C.dpre[NLAYER - 1], C.loss_partial, C.nx, C.ny,
-                nullptr, 0);
+                nullptr, 0, nullptr);
     backward(C, s);
     NEMO_HIP_CHECK(hipStreamSynchronize(s));  // t goes out of scope
 }
