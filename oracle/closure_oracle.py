@@ -6,7 +6,7 @@ the GPU kernels; nothing in the product calls it.
 The network (no reference counterpart; the learned closure is a project
 extension): four 5x5 convolutions with zero padding 2 on an ny x nx grid,
 channels 4 -> 32 -> 32 -> 32 -> 3 (stored padded as 16 -> 32 -> 32 -> 32 -> 16),
-GELU in erf form after the first three, mean squared error over the three
+GELU in tanh form after the first three, mean squared error over the three
 output channels. Parameters follow the GPU layout: per layer weights
 [25 taps][cout][cin] (tap = 5 dy + dx) then bias [cout].
 
@@ -19,7 +19,6 @@ differences.
 from __future__ import annotations
 
 import numpy as np
-from scipy.special import erf
 
 TAPS = 25
 LAYERS = ((16, 32), (32, 32), (32, 32), (32, 16))   # (cin, cout) as stored
@@ -57,12 +56,18 @@ def bf16(x: np.ndarray) -> np.ndarray:
     return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
+GELU_K = np.sqrt(2.0 / np.pi)
+GELU_C = 0.044715
+
+
 def gelu(x):
-    return 0.5 * x * (1.0 + erf(x / np.sqrt(2.0)))
+    """GELU, tanh form: 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3)))."""
+    return 0.5 * x * (1.0 + np.tanh(GELU_K * (x + GELU_C * x ** 3)))
 
 
 def gelu_grad(x):
-    return 0.5 * (1.0 + erf(x / np.sqrt(2.0))) + x * np.exp(-0.5 * x * x) / np.sqrt(2.0 * np.pi)
+    t = np.tanh(GELU_K * (x + GELU_C * x ** 3))
+    return 0.5 * (1.0 + t) + 0.5 * x * (1.0 - t * t) * GELU_K * (1.0 + 3.0 * GELU_C * x * x)
 
 
 def conv(x: np.ndarray, w: np.ndarray, b: np.ndarray | None) -> np.ndarray:

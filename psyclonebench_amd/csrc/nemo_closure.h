@@ -3,9 +3,12 @@
 // bf16 MFMA (FP32 accumulate), trained on the GPU, and the plumbing between
 // the fine and the coarse NEMOLite2D states (nemo_closure.hip).
 //
-// Activation tensors are NHWC bf16 with a 2-pixel zero border: pixel (y, x)
-// of the ny x nx interior (x fastest) at ((y+2)*(nx+4) + x+2)*C + c. The
-// border is never written, so the 5x5 stencil needs no bounds tests.
+// Activation tensors are bf16 on a padded grid with a 2-pixel zero border:
+// padded pixel P = (y+2)*(nx+4) + x+2 for interior pixel (y, x), x fastest.
+// Hidden activations and gradients are stored in planes of 16 channels
+// (channel c at (c / 16) * plane + P * 16 + c % 16, plane = (ny+4)(nx+4)*16);
+// the network input holds its IN0C channels at P * IN0C + c. The border is
+// never written, so the 5x5 stencil needs no bounds tests.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -15,7 +18,8 @@ namespace cl {
 
 constexpr int TAPS = 25;   // 5 x 5 kernel
 constexpr int HID = 32;    // hidden channels
-constexpr int CIN0 = 16;   // input channels as stored (4 used: u, v, ssh, depth)
+constexpr int CIN0 = 16;   // input channels of the layer-0 weights (4 used: u, v, ssh, depth)
+constexpr int IN0C = 4;    // channels of the stored network input
 constexpr int COUT4 = 16;  // output channels as computed (3 used: du, dv, dssh)
 constexpr int NLAYER = 4;
 
@@ -83,7 +87,7 @@ void launch_optimizer(hipStream_t s, int kind, float *w, const float *g, float *
 // transposed copy for the input-gradient convolution [TAPS][cin][cout]
 void launch_weights_bf16(hipStream_t s, const float *w, int cin, int cout,
                          __bf16 *fwd, __bf16 *flip);
-// coarse state -> padded NHWC input tensor (CIN0 channels)
+// coarse state -> padded network input (IN0C channels) and the activity gate
 void launch_pack(hipStream_t s, const CoarseState &c, const Norm &nm, __bf16 *in0,
                  float *gate, int nx, int ny);
 // target[(y*nx+x)*4 + k] = (F_k - C_k) / out_scale_k, k = u, v, ssh
@@ -114,13 +118,13 @@ void launch_set_fields(hipStream_t s, int mode, const double *sssh, const double
 // Face ssh from the T-cell ssh (next_sshu/v over their regions), when *flag.
 void launch_face_ssh(hipStream_t s, const int *flag, const CoarseState &c, int nx, int ny);
 // Energy guard: E = sum over interior cells of g*ssh^2 + ht*(u^2 + v^2),
-// before and after the correction, and a non-finite count.
+// before and after the correction, and a non-finite count, as per-workgroup
+// partial sums (partial[block][3]); a second launch sets
+// flag = finite && E_after <= (1 + rel) * E_before + abs and counts trips.
 int guard_blocks();
-void launch_guard_partial(hipStream_t s, const float *pred, const Norm &nm,
-                          const CoarseState &c, double g, double *partial, int nx, int ny);
-// flag = finite && E_after <= (1 + rel) * E_before + abs; counts trips.
-void launch_guard_decide(hipStream_t s, const double *partial, int nblk, double rel,
-                         double abs_tol, int *flag, int *trips);
+void launch_guard(hipStream_t s, const float *pred, const Norm &nm, const CoarseState &c,
+                  double g, double *partial, double rel, double abs_tol, int *flag,
+                  int *trips, int nx, int ny);
 
 }  // namespace cl
 }  // namespace nemo

@@ -2,21 +2,21 @@
 // accumulation for the forward pass, the input gradient and the weight
 // gradient; reductions, the optimizer, and the fine <-> coarse plumbing.
 //
-// Convolution as an implicit GEMM: D[co][px] = sum_k W[co][k] * X[k][px],
-// k = (tap, ci). One wavefront computes 32 output channels x 32 pixels of one
-// row with v_mfma_f32_32x32x16_bf16 (16 input channels of one tap per
-// instruction), or 16 channels x 2 x 16 pixels with v_mfma_f32_16x16x32_bf16
-// (the 16-channel output layer, 32 input channels per instruction). Operand
-// fragments are 16-byte loads: weights [tap][co][ci] give a lane 8
-// consecutive ci of one co; NHWC activations give a lane 8 consecutive ci of
-// one pixel. The accumulator leaves each lane one pixel and groups of 4
-// consecutive channels, stored as 8-byte NHWC vectors.
+// Convolution as an implicit GEMM on v_mfma_f32_32x32x16_bf16 (32 output
+// channels x 32 pixels x 16 input channels): the weights sit in LDS for the
+// whole persistent launch, and one activation fragment (32 pixels of one
+// kernel row) feeds the five horizontal taps of that row through five
+// accumulators that are shifted into place at the end (k_conv_sh). The
+// output layer packs (tap column, channel) into the 16 rows of
+// v_mfma_f32_16x16x32_bf16 (k_conv_out). Activation fragments are 16-byte
+// loads of 8 consecutive channels of one pixel from a 16-channel plane.
 //
 // Weight gradient: dW[tap][co][ci] = sum_px dPre[px][co] * X[px + tap][ci],
 // a GEMM whose reduction runs over pixels, on v_mfma_f32_16x16x32_bf16 with
 // fragments gathered pixel by pixel; every workgroup sums its pixels into a
 // partial dW, and a second pass adds the partials in a fixed order.
 #include <cmath>
+#include <type_traits>
 
 #include "nemo_closure.h"
 #include "nemo_hip_common.h"
@@ -31,15 +31,28 @@ typedef __attribute__((ext_vector_type(4))) __bf16 bf16x4;
 typedef __attribute__((ext_vector_type(16))) float f32x16;
 typedef __attribute__((ext_vector_type(4))) float f32x4;
 
-constexpr float kInvSqrt2 = 0.70710678118654752f;
-constexpr float kInvSqrt2Pi = 0.39894228040143268f;
+// GELU in its tanh form: 0.5 x (1 + tanh(u / 2)) = x s(u), s the logistic
+// sigmoid, u = 2 sqrt(2 / pi) (x + 0.044715 x^3). One hardware exp2 and one
+// reciprocal, no branches: the epilogues evaluate 32 of these per pixel and
+// layer. exp(-u) = exp2(x (A + B x^2)).
+constexpr float kLog2e = 1.44269504088896341f;
+constexpr float kGeluK = 2.0f * 0.79788456080286536f;  // 2 sqrt(2 / pi)
+constexpr float kGeluC = 0.044715f;
+constexpr float kGeluA = -kGeluK * kLog2e;
+constexpr float kGeluB = kGeluA * kGeluC;
 
-// GELU, erf form: 0.5 x (1 + erf(x / sqrt 2)), and its derivative
-// Phi(x) + x phi(x).
-__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * kInvSqrt2)); }
+__device__ __forceinline__ float gelu_sigmoid(float x)
+{
+    const float e = __builtin_amdgcn_exp2f(x * fmaf(x * x, kGeluB, kGeluA));  // exp(-u)
+    return __builtin_amdgcn_rcpf(1.0f + e);
+}
+__device__ __forceinline__ float gelu(float x) { return x * gelu_sigmoid(x); }
+// d/dx x s(u) = s + x s (1 - s) du/dx, du/dx = K (1 + 3 C x^2)
 __device__ __forceinline__ float gelu_grad(float x)
 {
-    return 0.5f * (1.0f + erff(x * kInvSqrt2)) + x * expf(-0.5f * x * x) * kInvSqrt2Pi;
+    const float s = gelu_sigmoid(x);
+    const float du = kGeluK * fmaf(3.0f * kGeluC, x * x, 1.0f);
+    return fmaf(x * s * (1.0f - s), du, s);
 }
 
 __device__ __forceinline__ size_t pad_index(int y, int x, int nx)
@@ -55,168 +68,386 @@ __device__ __forceinline__ size_t plane_elems(int nx, int ny)
     return (size_t)(ny + 4) * (size_t)(nx + 4) * 16;
 }
 
-// Weights live in LDS for the whole launch (rows of cin bf16 padded by 8 so
-// that the 16-byte fragment reads of 16 lanes spread over the banks);
-// activations come from global memory through the L1, where a 32-pixel row
-// segment serves all 25 taps of its tile. Each wavefront computes two tiles
-// per iteration, so every weight fragment read from LDS feeds two MFMAs.
-template <int CIN, int COUT, int EPI>
-__global__ __launch_bounds__(256) void k_conv(ConvArgs a)
+
+// ---- shift-accumulate convolutions ------------------------------------------
+//
+// One activation fragment (32 input pixels of one kernel row, 16 channels)
+// feeds the five horizontal taps of that row: accumulator dx collects
+// D_dx[co][n] = sum over dy, ci of W[dy][dx][co][ci] * X[y + dy][x0 + n][ci]
+// (padded coordinates), and output pixel x0 + j is sum over dx of
+// D_dx[co][j + dx], formed after the reduction by four DPP wavefront shifts.
+// A tile of 32 input pixels therefore yields 28 outputs, and the activation
+// traffic per output is a fifth of one load per tap. Tiles sit at
+// x0 = min(28 t, nx - 28); the last one overlaps its neighbour and stores
+// only the pixels x >= 28 t.
+//
+// Work items (row y, tile t) are split into eight contiguous bands, one per
+// XCD (workgroups are dispatched to the XCDs round robin), so that the five
+// kernel rows an output row reads come from its own XCD's L2.
+
+// lane i <- lane i + 1 over the whole wavefront (DPP wave_shl:1); lane 63 <- 0
+__device__ __forceinline__ float wave_shl1(float v)
 {
-    static_assert(CIN == 16 || CIN == 32, "input channels");
-    static_assert((COUT == 32) || (COUT == 16 && CIN == 32), "output channels");
-    constexpr int ROW = CIN + 8;
-    __shared__ __attribute__((aligned(16))) __bf16 wsh[TAPS * COUT * ROW];
-    for (int q = (int)threadIdx.x; q < TAPS * COUT * (CIN / 8); q += (int)blockDim.x) {
-        const int r = q / (CIN / 8), c8 = q - r * (CIN / 8);
-        *reinterpret_cast<bf16x8 *>(&wsh[r * ROW + c8 * 8]) =
-            *reinterpret_cast<const bf16x8 *>(a.w + (size_t)r * CIN + c8 * 8);
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xf, 0xf, true));
+}
+
+// Items [first, first + count) of this workgroup's XCD band; the workgroup is
+// number `local` of `nlocal` on that band. gridDim.x is a multiple of 8.
+struct Band {
+    int first, count, local, nlocal;
+};
+__device__ __forceinline__ Band xcd_band(int items)
+{
+    const int xcd = (int)(blockIdx.x & 7);
+    const int first = (int)((long)items * xcd / 8);
+    const int last = (int)((long)items * (xcd + 1) / 8);
+    return Band{first, last - first, (int)(blockIdx.x >> 3), (int)(gridDim.x >> 3)};
+}
+
+// 32 output channels. KIND 0: the network input (4 channels, 8 bytes a
+// pixel), K packed as (kernel row, channel): chunk 0 holds rows 0..3, chunk 1
+// row 4 and zeros. KIND 1: 16 input channels. KIND 2: 32 input channels.
+// EPI_HIDDEN: bias, GELU, optional pre-activation; EPI_DX: times gelu'(gpre).
+// Two wavefronts per SIMD (at most 256 registers): two workgroups share a CU.
+template <int KIND, int EPI>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_conv_sh(
+    ConvArgs a)
+{
+    static_assert(EPI == EPI_HIDDEN || (EPI == EPI_DX && KIND != 0), "epilogue");
+    constexpr int NC = KIND == 2 ? 2 : 1;
+    constexpr int CIN = KIND == 2 ? 32 : 16;
+    constexpr int ROW = CIN + 8;  // LDS row of one output channel, padded by 8
+    constexpr int NROWS = KIND == 0 ? 5 * 2 * 32 : TAPS * 32;
+    __shared__ __attribute__((aligned(16))) __bf16 wsh[NROWS * ROW];
+    // Weight fill (256 threads): every global load is issued before the first
+    // LDS store, so the workgroup waits for one memory latency, not one per
+    // loop trip.
+    const int tid = (int)threadIdx.x;
+    if constexpr (KIND == 0) {
+        // wsh[(dx * 2 + chunk) * 32 + co][4 kq .. 4 kq + 3] from the [TAPS][32][16]
+        // weights: chunk 0 holds kernel row kq, chunk 1 row 4 (kq = 0) and zeros
+        constexpr int NQ = 5 * 2 * 32 * 4 / 256;
+        bf16x4 v[NQ];
+#pragma unroll
+        for (int i = 0; i < NQ; i++) {
+            const int q = tid + 256 * i;
+            const int kq = q & 3, co = (q >> 2) & 31, ch = (q >> 7) & 1, dx = q >> 8;
+            const int dy = ch == 0 ? kq : 4;
+            v[i] = (ch == 0 || kq == 0)
+                       ? *reinterpret_cast<const bf16x4 *>(a.w + ((size_t)(dy * 5 + dx) * 32 + co) * 16)
+                       : bf16x4{};
+        }
+#pragma unroll
+        for (int i = 0; i < NQ; i++) {
+            const int q = tid + 256 * i;
+            const int kq = q & 3, co = (q >> 2) & 31, ch = (q >> 7) & 1, dx = q >> 8;
+            *reinterpret_cast<bf16x4 *>(&wsh[((dx * 2 + ch) * 32 + co) * ROW + kq * 4]) = v[i];
+        }
+    } else {
+        constexpr int NTOT = TAPS * 32 * (CIN / 8), NQ = (NTOT + 255) / 256;
+        bf16x8 v[NQ];
+#pragma unroll
+        for (int i = 0; i < NQ; i++) {
+            const int q = tid + 256 * i;
+            if (q < NTOT) v[i] = *reinterpret_cast<const bf16x8 *>(a.w + (size_t)q * 8);
+        }
+#pragma unroll
+        for (int i = 0; i < NQ; i++) {
+            const int q = tid + 256 * i;
+            const int r = q / (CIN / 8), c8 = q - r * (CIN / 8);
+            if (q < NTOT) *reinterpret_cast<bf16x8 *>(&wsh[r * ROW + c8 * 8]) = v[i];
+        }
     }
     __syncthreads();
 
-    const int lane = (int)(threadIdx.x & 63);
-    const int wave = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
-    const int nwaves = (int)(gridDim.x * (blockDim.x >> 6));
-    const int nx = a.nx, Wp = nx + 4;
-    const int ntx = nx >> 5, items = a.ny * ntx;
-    const size_t PL = plane_elems(nx, a.ny);
+    const int lane = (int)(threadIdx.x & 63), n = lane & 31, h = lane >> 5;
+    const int nx = a.nx, Wp = nx + 4, T = (nx + 27) / 28;
+    constexpr int PXS = KIND == 0 ? IN0C : 16;  // elements per padded pixel of a plane
+    const size_t PL = plane_elems(nx, a.ny), RS = (size_t)Wp * PXS;  // row stride
+    const Band bd = xcd_band(a.ny * T);
+    const int wib = bd.local * 4 + (int)(threadIdx.x >> 6), nwb = bd.nlocal * 4;
 
-    // hidden / input-gradient epilogue of one 32-channel x 32-pixel tile
-    auto store32 = [&](const f32x16 &acc, int y, int x0) {
-        const size_t px = pad_index(y, x0 + (lane & 31), nx) * 16;
+    // (the lambdas are forced inline: an out-of-line call would move the
+    // accumulators to scratch memory)
+    // tile (row y, column tile t): items are y * T + t
+    struct Pos {
+        int y, t;
+    };
+    auto next_pos = [&](Pos p, int dy, int dt) __attribute__((always_inline)) {
+        p.y += dy;
+        p.t += dt;
+        if (p.t >= T) {
+            p.t -= T;
+            p.y++;
+        }
+        return p;
+    };
+    auto finish = [&](f32x16(&acc)[5], Pos p) __attribute__((always_inline)) {
+        const int y = p.y, t = p.t, x0 = min(28 * t, nx - 28);
+        f32x16 s = acc[4];
+#pragma unroll
+        for (int dx = 3; dx >= 0; dx--)
+#pragma unroll
+            for (int r = 0; r < 16; r++) s[r] = wave_shl1(s[r]) + acc[dx][r];
+        const int x = x0 + n;
+        if (n >= 28 || x < 28 * t) return;
+        const size_t px = pad_index(y, x, nx) * 16;
 #pragma unroll
         for (int g = 0; g < 4; g++) {
-            const int co = g * 8 + (lane >> 5) * 4;
+            const int co = g * 8 + h * 4;
             const size_t o = (size_t)(co >> 4) * PL + px + (co & 15);
-            float v[4] = {acc[4 * g], acc[4 * g + 1], acc[4 * g + 2], acc[4 * g + 3]};
             bf16x4 r;
             if constexpr (EPI == EPI_HIDDEN) {
                 bf16x4 p;
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
-                    v[k] += a.bias[co + k];
-                    p[k] = (__bf16)v[k];
-                    r[k] = (__bf16)gelu(v[k]);
+                    const float v = s[4 * g + k] + a.bias[co + k];
+                    p[k] = (__bf16)v;
+                    r[k] = (__bf16)gelu(v);
                 }
                 if (a.pre) *reinterpret_cast<bf16x4 *>(a.pre + o) = p;
-            } else {  // EPI_DX: gradient times gelu'(pre) of the layer below
+            } else {
                 const bf16x4 gp = *reinterpret_cast<const bf16x4 *>(a.gpre + o);
 #pragma unroll
-                for (int k = 0; k < 4; k++) r[k] = (__bf16)(v[k] * gelu_grad((float)gp[k]));
+                for (int k = 0; k < 4; k++)
+                    r[k] = (__bf16)(s[4 * g + k] * gelu_grad((float)gp[k]));
             }
             *reinterpret_cast<bf16x4 *>(a.out + o) = r;
         }
     };
-    // output-layer epilogue: used channels 0..2 sit in lanes 0..15
-    auto store16 = [&](const f32x4 &lo, const f32x4 &hi, int y, int x0) {
-        if ((lane >> 4) != 0) return;
-        float4 r0, r1;
-        r0.x = lo[0] + a.bias[0];
-        r0.y = lo[1] + a.bias[1];
-        r0.z = lo[2] + a.bias[2];
-        r0.w = 0.0f;
-        r1.x = hi[0] + a.bias[0];
-        r1.y = hi[1] + a.bias[1];
-        r1.z = hi[2] + a.bias[2];
-        r1.w = 0.0f;
-        const size_t p = (size_t)y * nx + x0 + (lane & 15);
-        if (a.gate) {
-            const float g0 = a.gate[p], g1 = a.gate[p + 16];
-            r0.x *= g0;
-            r0.y *= g0;
-            r0.z *= g0;
-            r1.x *= g1;
-            r1.y *= g1;
-            r1.z *= g1;
-        }
-        float4 *dst = reinterpret_cast<float4 *>(a.outf) + p;
-        dst[0] = r0;
-        dst[16] = r1;
+
+    // A wavefront takes the tile pairs at band offsets k = 2 wib, 2 (wib + nwb),
+    // ...; the activation fragments of the next pair are in flight while the
+    // current pair runs its MFMAs and epilogue.
+    int k = 2 * wib;
+    if (k >= bd.count) return;
+    const size_t coff = KIND == 0 ? 0 : (size_t)h * 8;
+    // positions of the pair at k, advanced by 2 nwb items per pair (no
+    // division in the loop)
+    const int step = 2 * nwb, step_y = step / T, step_t = step - step_y * T;
+    Pos pos{(bd.first + k) / T, (bd.first + k) - (bd.first + k) / T * T};
+    auto second = [&](Pos p, int kk) __attribute__((always_inline)) {
+        return kk + 1 < bd.count ? next_pos(p, 0, 1) : p;
+    };
+    auto pair_src = [&](Pos p, int kk, const __bf16 *&sa, const __bf16 *&sb)
+                        __attribute__((always_inline)) {
+        const Pos q = second(p, kk);
+        const int xa = min(28 * p.t, nx - 28), xb = min(28 * q.t, nx - 28);
+        sa = a.in + ((size_t)p.y * Wp + xa + n) * PXS + coff;
+        sb = a.in + ((size_t)q.y * Wp + xb + n) * PXS + coff;
+    };
+    const __bf16 *sa, *sb;
+    pair_src(pos, k, sa, sb);
+    f32x16 acc[2][5];
+    auto finish_pair = [&](Pos p, int kk) __attribute__((always_inline)) {
+        finish(acc[0], p);
+        if (kk + 1 < bd.count) finish(acc[1], next_pos(p, 0, 1));  // wavefront-uniform
     };
 
-    for (int it = 2 * wave; it < items; it += 2 * nwaves) {
-        const bool two = it + 1 < items;  // wavefront-uniform
-        const int ya = it / ntx, xa = (it - ya * ntx) * 32;
-        const int it2 = two ? it + 1 : it;
-        const int yb = it2 / ntx, xb = (it2 - yb * ntx) * 32;
-        // The activation fragments of one kernel row (dy) are loaded one row
-        // ahead of the MFMAs that use them (two register buffers), so a
-        // wavefront keeps a whole row's loads in flight.
-        if constexpr (COUT == 32) {
-            constexpr int NC = CIN / 16;
-            const int pl = lane & 31, kg = (lane >> 5) * 8;
-            f32x16 acc0 = {}, acc1 = {};
-            const __bf16 *sa = a.in + (size_t)(ya * Wp + xa + pl) * 16 + kg;
-            const __bf16 *sb = a.in + (size_t)(yb * Wp + xb + pl) * 16 + kg;
-            bf16x8 buf[2][5][NC][2];
-            auto load_row = [&](int dy, int slot) {
+    if constexpr (KIND == 0) {
+        // rows 2h, 2h + 1 and 4 of each tile, channels 0..3
+        bf16x4 raw[2][3];
+        auto load = [&](const __bf16 *pa, const __bf16 *pb) __attribute__((always_inline)) {
+            raw[0][0] = *reinterpret_cast<const bf16x4 *>(pa + (2 * h) * RS);
+            raw[0][1] = *reinterpret_cast<const bf16x4 *>(pa + (2 * h + 1) * RS);
+            raw[0][2] = *reinterpret_cast<const bf16x4 *>(pa + 4 * RS);
+            raw[1][0] = *reinterpret_cast<const bf16x4 *>(pb + (2 * h) * RS);
+            raw[1][1] = *reinterpret_cast<const bf16x4 *>(pb + (2 * h + 1) * RS);
+            raw[1][2] = *reinterpret_cast<const bf16x4 *>(pb + 4 * RS);
+        };
+        load(sa, sb);
+        for (; k < bd.count; k += 2 * nwb) {
+            const Pos cur = pos;
+            const bf16x4 z = {};
+            bf16x8 B0[2], B1[2];
 #pragma unroll
-                for (int dx = 0; dx < 5; dx++)
-#pragma unroll
-                    for (int c = 0; c < NC; c++) {
-                        const size_t o = (size_t)c * PL + (size_t)(dy * Wp + dx) * 16;
-                        buf[slot][dx][c][0] = *reinterpret_cast<const bf16x8 *>(sa + o);
-                        buf[slot][dx][c][1] = *reinterpret_cast<const bf16x8 *>(sb + o);
-                    }
-            };
-            load_row(0, 0);
-#pragma unroll
-            for (int dy = 0; dy < 5; dy++) {
-                if (dy < 4) load_row(dy + 1, (dy + 1) & 1);
-                // keep the next row's loads ahead of this row's MFMAs
-                __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                for (int dx = 0; dx < 5; dx++) {
-                    const __bf16 *ws = wsh + ((dy * 5 + dx) * 32 + pl) * ROW + kg;
-#pragma unroll
-                    for (int c = 0; c < NC; c++) {
-                        const bf16x8 w = *reinterpret_cast<const bf16x8 *>(ws + c * 16);
-                        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, buf[dy & 1][dx][c][0],
-                                                                       acc0, 0, 0, 0);
-                        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, buf[dy & 1][dx][c][1],
-                                                                       acc1, 0, 0, 0);
-                    }
-                }
+            for (int j = 0; j < 2; j++) {
+                B0[j] = __builtin_shufflevector(raw[j][0], raw[j][1], 0, 1, 2, 3, 4, 5, 6, 7);
+                B1[j] = __builtin_shufflevector(h ? z : raw[j][2], z, 0, 1, 2, 3, 4, 5, 6, 7);
             }
-            store32(acc0, ya, xa);
-            if (two) store32(acc1, yb, xb);
-        } else {
-            const int pl = lane & 15, kg = (lane >> 4) * 8;
-            f32x4 a0 = {}, a1 = {}, b0 = {}, b1 = {};
-            // lane k-group kg covers channels kg..kg+7: plane kg / 16
-            const size_t kof = (size_t)(kg >> 4) * PL + (kg & 15);
-            const __bf16 *sa = a.in + kof + (size_t)(ya * Wp + xa + pl) * 16;
-            const __bf16 *sb = a.in + kof + (size_t)(yb * Wp + xb + pl) * 16;
-            bf16x8 buf[2][5][4];
-            auto load_row = [&](int dy, int slot) {
+            if (k + 2 * nwb < bd.count) {
+                pos = next_pos(pos, step_y, step_t);
+                pair_src(pos, k + 2 * nwb, sa, sb);
+                load(sa, sb);
+            }
 #pragma unroll
-                for (int dx = 0; dx < 5; dx++) {
-                    const size_t o = (size_t)(dy * Wp + dx) * 16;
-                    buf[slot][dx][0] = *reinterpret_cast<const bf16x8 *>(sa + o);
-                    buf[slot][dx][1] = *reinterpret_cast<const bf16x8 *>(sa + o + 16 * 16);
-                    buf[slot][dx][2] = *reinterpret_cast<const bf16x8 *>(sb + o);
-                    buf[slot][dx][3] = *reinterpret_cast<const bf16x8 *>(sb + o + 16 * 16);
-                }
-            };
-            load_row(0, 0);
+            for (int j = 0; j < 2; j++)
 #pragma unroll
-            for (int dy = 0; dy < 5; dy++) {
-                if (dy < 4) load_row(dy + 1, (dy + 1) & 1);
-                __builtin_amdgcn_sched_barrier(0);
+                for (int dx = 0; dx < 5; dx++) acc[j][dx] = f32x16{};
+            // chunk-outer order: an accumulator comes back every 10 MFMAs
+#pragma unroll
+            for (int ch = 0; ch < 2; ch++)
 #pragma unroll
                 for (int dx = 0; dx < 5; dx++) {
                     const bf16x8 w = *reinterpret_cast<const bf16x8 *>(
-                        wsh + ((dy * 5 + dx) * 16 + pl) * ROW + kg);
-                    const int s = dy & 1;
-                    a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, buf[s][dx][0], a0, 0, 0, 0);
-                    a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, buf[s][dx][1], a1, 0, 0, 0);
-                    b0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, buf[s][dx][2], b0, 0, 0, 0);
-                    b1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, buf[s][dx][3], b1, 0, 0, 0);
+                        wsh + ((dx * 2 + ch) * 32 + n) * ROW + h * 8);
+                    acc[0][dx] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, ch ? B1[0] : B0[0],
+                                                                         acc[0][dx], 0, 0, 0);
+                    acc[1][dx] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, ch ? B1[1] : B0[1],
+                                                                         acc[1][dx], 0, 0, 0);
                 }
-            }
-            store16(a0, a1, ya, xa);
-            if (two) store16(b0, b1, yb, xb);
+            finish_pair(cur, k);
         }
+    } else {
+        // Two register slots of one kernel row each: row dy of a pair sits in
+        // slot (dy + P) & 1, and row 0 of the next pair is loaded with row 4
+        // of this one into slot (P + 1) & 1, so consecutive pairs alternate P.
+        bf16x8 buf[2][NC][2];
+        auto load_row = [&](const __bf16 *pa, const __bf16 *pb, int dy, int slot)
+                            __attribute__((always_inline)) {
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+                const size_t o = (size_t)c * PL + (size_t)dy * RS;
+                buf[slot][c][0] = *reinterpret_cast<const bf16x8 *>(pa + o);
+                buf[slot][c][1] = *reinterpret_cast<const bf16x8 *>(pb + o);
+            }
+        };
+        auto pair = [&](auto parity) __attribute__((always_inline)) {
+            constexpr int P = decltype(parity)::value;
+            const bool more = k + 2 * nwb < bd.count;  // wavefront-uniform
+            const __bf16 *na = sa, *nb = sb;
+            const Pos cur = pos;
+            if (more) {
+                pos = next_pos(pos, step_y, step_t);
+                pair_src(pos, k + 2 * nwb, na, nb);
+            }
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+#pragma unroll
+                for (int dx = 0; dx < 5; dx++) acc[j][dx] = f32x16{};
+#pragma unroll
+            for (int dy = 0; dy < 5; dy++) {
+                if (dy < 4)
+                    load_row(sa, sb, dy + 1, (dy + 1 + P) & 1);
+                else if (more)
+                    load_row(na, nb, 0, (P + 1) & 1);
+                __builtin_amdgcn_sched_barrier(0);
+                // channel-chunk-outer order: an accumulator comes back every
+                // 10 MFMAs, past the latency of the previous one
+#pragma unroll
+                for (int c = 0; c < NC; c++)
+#pragma unroll
+                    for (int dx = 0; dx < 5; dx++) {
+                        const bf16x8 w = *reinterpret_cast<const bf16x8 *>(
+                            wsh + ((dy * 5 + dx) * 32 + n) * ROW + h * 8 + c * 16);
+                        acc[0][dx] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                            w, buf[(dy + P) & 1][c][0], acc[0][dx], 0, 0, 0);
+                        acc[1][dx] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                            w, buf[(dy + P) & 1][c][1], acc[1][dx], 0, 0, 0);
+                    }
+            }
+            finish_pair(cur, k);
+            k += 2 * nwb;
+            sa = na;
+            sb = nb;
+        };
+        load_row(sa, sb, 0, 0);
+        while (true) {
+            pair(std::integral_constant<int, 0>{});
+            if (k >= bd.count) break;
+            pair(std::integral_constant<int, 1>{});
+            if (k >= bd.count) break;
+        }
+    }
+}
+
+// Output layer (32 -> 3 used channels): the 16 rows of v_mfma_f32_16x16x32_bf16
+// hold (dx, co) = dx * 3 + co, so one instruction covers a whole kernel row
+// of 16 input pixels and 32 channels; the shifted sum over dx goes through a
+// per-wavefront LDS tile. 16 input pixels give 12 outputs; four tiles per
+// wavefront and iteration.
+__global__ __launch_bounds__(256) void k_conv_out(ConvArgs a)
+{
+    constexpr int ROW = 40;
+    __shared__ __attribute__((aligned(16))) __bf16 wsh[5 * 16 * ROW];
+    __shared__ float dsh[4][4][16][17];  // [wave][tile][dx * 3 + co][pixel]
+    {
+        // 16-byte chunks c8 of row (dy, m); all loads before the LDS stores
+        constexpr int NTOT = 5 * 16 * 4, NQ = (NTOT + 255) / 256;
+        const int tid = (int)threadIdx.x;
+        bf16x8 v[NQ];
+#pragma unroll
+        for (int i = 0; i < NQ; i++) {
+            const int q = tid + 256 * i;
+            const int c8 = q & 3, m = (q >> 2) & 15, dy = q >> 6;
+            const int dx = m / 3, co = m - 3 * dx;
+            v[i] = (q < NTOT && m < 15) ? *reinterpret_cast<const bf16x8 *>(
+                                              a.w + ((size_t)(dy * 5 + dx) * COUT4 + co) * 32 + c8 * 8)
+                                        : bf16x8{};
+        }
+#pragma unroll
+        for (int i = 0; i < NQ; i++) {
+            const int q = tid + 256 * i;
+            const int c8 = q & 3, m = (q >> 2) & 15, dy = q >> 6;
+            if (q < NTOT) *reinterpret_cast<bf16x8 *>(&wsh[(dy * 16 + m) * ROW + c8 * 8]) = v[i];
+        }
+    }
+    __syncthreads();
+
+    const int lane = (int)(threadIdx.x & 63), wv = (int)(threadIdx.x >> 6);
+    const int n = lane & 15, q4 = lane >> 4, kg = q4 * 8;
+    const int nx = a.nx, Wp = nx + 4, T = (nx + 11) / 12;
+    const size_t PL = plane_elems(nx, a.ny), RS = (size_t)Wp * 16;
+    const Band bd = xcd_band(a.ny * T);
+    const int wib = bd.local * 4 + wv, nwb = bd.nlocal * 4;
+    const size_t kof = (size_t)(kg >> 4) * PL + (kg & 15);
+    float(*d)[16][17] = dsh[wv];
+
+    for (int k = 4 * wib; k < bd.count; k += 4 * nwb) {
+        const __bf16 *src[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int it = bd.first + min(k + j, bd.count - 1);
+            const int y = it / T, x0 = min(12 * (it - y * T), nx - 12);
+            src[j] = a.in + kof + ((size_t)y * Wp + x0 + n) * 16;
+        }
+        bf16x8 b[5][4];
+#pragma unroll
+        for (int dy = 0; dy < 5; dy++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                b[dy][j] = *reinterpret_cast<const bf16x8 *>(src[j] + dy * RS);
+        f32x4 acc[4] = {};
+#pragma unroll
+        for (int dy = 0; dy < 5; dy++) {
+            const bf16x8 w = *reinterpret_cast<const bf16x8 *>(wsh + (dy * 16 + n) * ROW + kg);
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, b[dy][j], acc[j], 0, 0, 0);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) d[j][q4 * 4 + r][n] = acc[j][r];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // lane: tile q4, output pixel n of the tile (n < 12 stored)
+        const int nc = min(n, 11);
+        float v[3];
+#pragma unroll
+        for (int co = 0; co < 3; co++) {
+            float s = 0.0f;
+#pragma unroll
+            for (int dx = 0; dx < 5; dx++) s += d[q4][dx * 3 + co][nc + dx];
+            v[co] = s + a.bias[co];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int it = bd.first + k + q4;
+        if (k + q4 >= bd.count || n >= 12) continue;
+        const int y = it / T, t = it - y * T, x = min(12 * t, nx - 12) + n;
+        if (x < 12 * t) continue;
+        const size_t p = (size_t)y * nx + x;
+        const float g = a.gate ? a.gate[p] : 1.0f;
+        float4 r;
+        r.x = v[0] * g;
+        r.y = v[1] * g;
+        r.z = v[2] * g;
+        r.w = 0.0f;
+        reinterpret_cast<float4 *>(a.outf)[p] = r;
     }
 }
 
@@ -230,6 +461,7 @@ __device__ __forceinline__ bf16x8 gather8(const __bf16 *p, size_t stride)
 
 // Partial weight gradient of one workgroup. Wave w handles the taps w,
 // w+4, ...; the dPre fragment of a 32-pixel chunk is shared by its taps.
+// CI == CIN0 is layer 0, whose input is the IN0C-channel network input.
 template <int CO, int CI>
 __global__ __launch_bounds__(256) void k_dw(const __bf16 *__restrict__ dpre,
                                             const __bf16 *__restrict__ in,
@@ -260,10 +492,16 @@ __global__ __launch_bounds__(256) void k_dw(const __bf16 *__restrict__ dpre,
             const int tap = wv + 4 * t;
             if (tap < TAPS) {
                 const int dy = tap / 5, dx = tap - 5 * (tap / 5);
-                const __bf16 *ip = in + ((size_t)(y + dy) * Wp + x0 + kg + dx) * 16 + pl;
+                const size_t pix = (size_t)(y + dy) * Wp + x0 + kg + dx;
 #pragma unroll
                 for (int nb = 0; nb < NB; nb++) {
-                    const bf16x8 B = gather8(ip + nb * PL, 16);
+                    bf16x8 B;
+                    if constexpr (CI == CIN0) {
+                        // the network input: IN0C channels a pixel, the others zero
+                        B = pl < IN0C ? gather8(in + pix * IN0C + pl, IN0C) : bf16x8{};
+                    } else {
+                        B = gather8(in + pix * 16 + pl + nb * PL, 16);
+                    }
 #pragma unroll
                     for (int mb = 0; mb < MB; mb++)
                         acc[t][mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
@@ -389,7 +627,7 @@ __global__ void k_pack(CoarseState c, Norm nm, __bf16 *__restrict__ in0,
     if (p >= nx * ny) return;
     const int y = p / nx, x = p - y * nx;
     const size_t k = (size_t)(y + 1) * c.W + (x + 1);
-    bf16x8 lo = {}, hi = {};
+    bf16x4 lo;
     const float u = ((float)c.u[k] - nm.in_shift[0]) / nm.in_scale[0];
     const float v = ((float)c.v[k] - nm.in_shift[1]) / nm.in_scale[1];
     const float s = ((float)c.ssh[k] - nm.in_shift[2]) / nm.in_scale[2];
@@ -401,9 +639,8 @@ __global__ void k_pack(CoarseState c, Norm nm, __bf16 *__restrict__ in0,
         const float act = (fabsf(u) + fabsf(v) + fabsf(s)) / nm.gate_eps;
         gate[p] = nm.gate_eps > 0.0f ? 1.0f - expf(-act * act) : 1.0f;
     }
-    bf16x8 *dst = reinterpret_cast<bf16x8 *>(in0 + pad_index(y, x, nx) * CIN0);
-    dst[0] = lo;
-    dst[1] = hi;
+    static_assert(IN0C == 4, "one 8-byte pixel");
+    *reinterpret_cast<bf16x4 *>(in0 + pad_index(y, x, nx) * IN0C) = lo;
 }
 
 __global__ void k_target(const double *__restrict__ fssh, const double *__restrict__ fu,
@@ -573,6 +810,9 @@ __global__ __launch_bounds__(256) void k_guard_partial(const float *__restrict__
     if (threadIdx.x < 3) partial[blockIdx.x * 3 + threadIdx.x] = red[threadIdx.x][0];
 }
 
+// Adds the partial sums in workgroup order (the same sum every call) and
+// decides. A separate launch rather than a last-workgroup epilogue: the
+// device-scope fence that epilogue needs writes back each XCD's L2.
 __global__ __launch_bounds__(256) void k_guard_decide(const double *__restrict__ partial,
                                                       int nblk, double rel, double abs_tol,
                                                       int *__restrict__ flag,
@@ -602,34 +842,40 @@ inline dim3 grid_for(long n, int block = 256)
     return dim3((unsigned)((n + block - 1) / block));
 }
 
-int conv_grid()
+int cu_count()
 {
-    static int g = [] {
+    static int n = [] {
         int dev = 0, cus = 0;
         NEMO_HIP_CHECK(hipGetDevice(&dev));
         NEMO_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        return cus * 2;  // 4 wavefronts per workgroup; 2 workgroups fit a CU's LDS
+        return cus;
     }();
-    return g;
+    return n;
 }
+
+// Persistent grid of `per_cu` 4-wavefront workgroups per CU, rounded up to a
+// multiple of 8 for the per-XCD bands.
+dim3 conv_grid(int per_cu) { return dim3((unsigned)((cu_count() * per_cu + 7) / 8 * 8)); }
 
 }  // namespace
 
 void launch_conv(hipStream_t s, int cin, int cout, Epilogue epi, const ConvArgs &a)
 {
-    if (a.nx % 32 != 0 || a.nx <= 0 || a.ny <= 0)
-        NEMO_FATAL("closure conv: nx = %d must be a positive multiple of 32", a.nx);
-    const dim3 grid((unsigned)conv_grid()), block(256);
-    if (cin == 16 && cout == 32 && epi == EPI_HIDDEN)
-        k_conv<16, 32, EPI_HIDDEN><<<grid, block, 0, s>>>(a);
+    if (a.nx < 32 || a.ny <= 0)
+        NEMO_FATAL("closure conv: interior %d x %d (nx must be at least 32)", a.nx, a.ny);
+    // k_conv_sh: 2 workgroups per CU (LDS weights and 256 registers);
+    // k_conv_out: 6 (24 KB of LDS, few registers)
+    const dim3 grid = conv_grid(2), block(256);
+    if (cin == 16 && cout == 32 && epi == EPI_HIDDEN)  // the network input: 4 used channels
+        k_conv_sh<0, EPI_HIDDEN><<<grid, block, 0, s>>>(a);
     else if (cin == 32 && cout == 32 && epi == EPI_HIDDEN)
-        k_conv<32, 32, EPI_HIDDEN><<<grid, block, 0, s>>>(a);
+        k_conv_sh<2, EPI_HIDDEN><<<grid, block, 0, s>>>(a);
     else if (cin == 32 && cout == 16 && epi == EPI_OUTPUT)
-        k_conv<32, 16, EPI_OUTPUT><<<grid, block, 0, s>>>(a);
+        k_conv_out<<<conv_grid(6), block, 0, s>>>(a);
     else if (cin == 16 && cout == 32 && epi == EPI_DX)
-        k_conv<16, 32, EPI_DX><<<grid, block, 0, s>>>(a);
+        k_conv_sh<1, EPI_DX><<<grid, block, 0, s>>>(a);
     else if (cin == 32 && cout == 32 && epi == EPI_DX)
-        k_conv<32, 32, EPI_DX><<<grid, block, 0, s>>>(a);
+        k_conv_sh<2, EPI_DX><<<grid, block, 0, s>>>(a);
     else
         NEMO_FATAL("closure conv: unsupported %d -> %d channels, epilogue %d", cin, cout,
                    (int)epi);
@@ -745,20 +991,19 @@ void launch_face_ssh(hipStream_t s, const int *flag, const CoarseState &c, int n
     NEMO_LAUNCH_CHECK();
 }
 
-int guard_blocks() { return 512; }
+// about one point per thread on a 1024 x 1024 coarse grid, so that every
+// thread's loads are in flight at once
+int guard_blocks() { return 4096; }
 
-void launch_guard_partial(hipStream_t s, const float *pred, const Norm &nm,
-                          const CoarseState &c, double g, double *partial, int nx, int ny)
+void launch_guard(hipStream_t s, const float *pred, const Norm &nm, const CoarseState &c,
+                  double g, double *partial, double rel, double abs_tol, int *flag,
+                  int *trips, int nx, int ny)
 {
     k_guard_partial<<<dim3((unsigned)guard_blocks()), dim3(256), 0, s>>>(pred, nm, c, g,
                                                                          partial, nx, ny);
     NEMO_LAUNCH_CHECK();
-}
-
-void launch_guard_decide(hipStream_t s, const double *partial, int nblk, double rel,
-                         double abs_tol, int *flag, int *trips)
-{
-    k_guard_decide<<<dim3(1), dim3(256), 0, s>>>(partial, nblk, rel, abs_tol, flag, trips);
+    k_guard_decide<<<dim3(1), dim3(256), 0, s>>>(partial, guard_blocks(), rel, abs_tol, flag,
+                                                 trips);
     NEMO_LAUNCH_CHECK();
 }
 

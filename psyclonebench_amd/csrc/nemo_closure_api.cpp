@@ -172,11 +172,11 @@ __bf16 to_bf16(float f)
 // normalised host inputs [ny*nx][4] -> C.in0
 void upload_inputs(ClosureContext &C, const float *in4, hipStream_t s)
 {
-    std::vector<__bf16> t(C.padded(CIN0), to_bf16(0.0f));
+    std::vector<__bf16> t(C.padded(IN0C), to_bf16(0.0f));
     for (int y = 0; y < C.ny; y++)
         for (int x = 0; x < C.nx; x++)
-            for (int k = 0; k < 4; k++)
-                t[((size_t)(y + 2) * (C.nx + 4) + x + 2) * CIN0 + k] =
+            for (int k = 0; k < IN0C; k++)
+                t[((size_t)(y + 2) * (C.nx + 4) + x + 2) * IN0C + k] =
                     to_bf16(in4[((size_t)y * C.nx + x) * 4 + k]);
     NEMO_HIP_CHECK(hipMemcpyAsync(C.in0, t.data(), t.size() * sizeof(__bf16),
                                   hipMemcpyHostToDevice, s));
@@ -220,7 +220,7 @@ void *nemo_cl_create(int nx, int ny, int coarse_width, int coarse_height,
         C->wx[l] = zeros<__bf16>(layer_wsize(l), s);
         C->dpre[l] = zeros<__bf16>(C->padded(layer_cout(l)), s);
     }
-    C->in0 = zeros<__bf16>(C->padded(CIN0), s);
+    C->in0 = zeros<__bf16>(C->padded(IN0C), s);
     for (int l = 0; l < NLAYER - 1; l++) {
         C->act[l] = zeros<__bf16>(C->padded(HID), s);
         C->pre[l] = zeros<__bf16>(C->padded(HID), s);
@@ -403,9 +403,8 @@ void nemo_cl_correct(void *h, int guard, double *ssh, double *u, double *v, doub
     launch_pack(s, c, C.nm, C.in0, C.gate, C.nx, C.ny);
     forward(C, s, false, true);
     if (guard) {
-        launch_guard_partial(s, C.pred, C.nm, c, C.g, C.guard_partial, C.nx, C.ny);
-        launch_guard_decide(s, C.guard_partial, guard_blocks(), C.guard_rel, C.guard_abs,
-                            C.flag, C.trips);
+        launch_guard(s, C.pred, C.nm, c, C.g, C.guard_partial, C.guard_rel, C.guard_abs,
+                     C.flag, C.trips, C.nx, C.ny);
     }
     const int *flag = guard ? C.flag : nullptr;
     launch_set_fields(s, 1, nullptr, nullptr, nullptr, C.pred, C.nm, flag, c, C.nx, C.ny);

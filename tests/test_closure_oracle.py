@@ -48,9 +48,15 @@ def test_conv_gradients_are_adjoints():
 
 
 def test_gelu_values_and_derivative():
+    import torch
+    x = np.linspace(-9, 9, 721)
+    # the tanh form as torch implements it, independently of the oracle
+    ref = torch.nn.functional.gelu(torch.from_numpy(x), approximate="tanh").numpy()
+    np.testing.assert_allclose(co.gelu(x), ref, rtol=1e-13, atol=1e-15)
     assert co.gelu(0.0) == 0.0
-    assert np.isclose(co.gelu(1.0), 0.8413447460685429, rtol=1e-15)    # Phi(1)
-    assert np.isclose(co.gelu(-1.0), -0.15865525393145707, rtol=1e-14)  # -(1 - Phi(1))
+    # within 5e-4 of the erf form x Phi(x): Phi(1) = 0.8413447460685429
+    assert abs(co.gelu(1.0) - 0.8413447460685429) < 5e-4
+    assert abs(co.gelu(-1.0) + 0.15865525393145707) < 5e-4
     assert np.isclose(co.gelu(8.0), 8.0, rtol=1e-14)
     x = np.linspace(-5, 5, 101)
     h = 1e-6
