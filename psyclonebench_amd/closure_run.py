@@ -7,14 +7,21 @@
 
 Training: for every bathymetry seed (the fine configuration is the 4096^2
 FP64 workload with that seed: depth 100 m + 50 m relief, open west edge, 1 m
-tide, dt 20 s, dx 1 km), the fine model runs `factor` steps per coarse step;
-the coarse model (dx 4 km, dt 80 s, box-filtered depth) takes one plain step
-to C_k+1 and the network takes one optimizer step on the pair (input C_k+1,
-target F_k+1 - C_k+1, F the box-filtered fine state). The coarse model then
-continues from its own corrected state (as at inference, energy guard on),
-and is reset to F_k+1 every r steps, r growing from 1 to --rollout over the
-run, so that the network learns to pull drifted states back toward the fine
-solution. Normalisation scales come from a pilot run of the first seed.
+tide, dt 20 s, dx 1 km), the fine model first spins up for a random
+0 .. --max-spin coarse intervals, so that the training windows cover the
+tidal cycle; the coarse model starts from the box filter F of that fine
+state. Then, per coarse step, the fine model runs `factor` steps, the coarse
+model (dx 4 km, dt 80 s, box-filtered depth) takes one plain step to C_k+1,
+and the network takes one Adam step (cosine learning rate, optional global
+norm clipping) on the pair (input C_k+1, target F_k+1 - C_k+1). The coarse
+model then continues from its own corrected state (as at inference, energy
+guard on), and is reset to F_k+1 every r steps, r growing from 1 to
+--rollout over the run, so that the network learns to pull drifted states
+back toward the fine solution. The loss and the correction leave out the
+ssh of cells the step clamps to the tide, and an activity gate
+1 - exp(-(a / --gate-eps)^2), a = |u| + |v| + |ssh| in input units, scales
+the output so that water at rest gets no correction. Normalisation scales
+come from a pilot run of the first seed.
 
 Evaluation: for every held-out seed, the fine run, the plain coarse run and
 the coarse run with the learned correction (and the energy guard) advance
@@ -220,10 +227,13 @@ def load_weights(path) -> tuple[np.ndarray, dict]:
 
 
 def evaluate(n: int, factor: int, bathy: list[int], steps: int, every: int, weights,
-             guard_rel: float = 0.05, log_fn=log, trace_energy: bool = False) -> dict:
+             guard_rel: float | None = None, log_fn=log, trace_energy: bool = False) -> dict:
+    """guard_rel None: the energy-guard fraction the weights were trained with."""
     params, meta = load_weights(weights)
     norm = meta["norm"]
-    out = {"seeds": {}, "every": every, "steps": steps}
+    if guard_rel is None:
+        guard_rel = float(meta.get("guard_rel", 0.25))
+    out = {"seeds": {}, "every": every, "steps": steps, "guard_rel": guard_rel}
     for b in bathy:
         fine, coarse = pair(n, factor, b, steps)
         plain = coarse.copy()
@@ -360,7 +370,8 @@ def main(argv=None) -> None:
     e.add_argument("--every", type=int, default=100)
     e.add_argument("--weights", default=str(DEFAULT_WEIGHTS))
     e.add_argument("--json", default=None)
-    e.add_argument("--guard-rel", type=float, default=0.05)
+    e.add_argument("--guard-rel", type=float, default=None,
+                   help="energy-guard fraction (default: the one used in training)")
     e.add_argument("--trace-energy", action="store_true",
                    help="record the energy change of every correction (synchronises)")
     s = sub.add_parser("speed")
