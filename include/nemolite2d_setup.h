@@ -84,6 +84,27 @@ double nemo_field_checksum(const double *x, int W, int x0, int x1, int y0,
  * n uniforms in [0,1) from `seed`, 53-bit resolution. */
 void nemo_uniforms(uint64_t seed, int64_t n, double *out);
 
+/* ---- driver files (nemo_io.cpp) ---------------------------------------- */
+
+/* One Fortran Ew.d field (w <= 32, d <= 17) into out[0..w), no terminator. */
+void nemo_fortran_e(double x, int w, int d, char *out);
+
+/* go2d_<istp>_<rank>.dat of common/gocean2d_io_mod.f90:98-149 (model_write)
+ * to `path`: per internal T point (ji = xs..xe fastest, jj = ys..ye, 0-based)
+ * '(6e16.7)' of x = ji*dx, y = jj*dy, ht, sshn, 0.5*(un(ji-1)+un(ji)),
+ * 0.5*(vn(jj-1)+vn(jj)), and an empty record after each row. Returns 0, or
+ * -1 when the file cannot be written. */
+int nemo_write_results(const char *path, const double *ht, const double *sshn,
+                       const double *un, const double *vn, int width, int xs, int xe,
+                       int ys, int ye, double dx, double dy);
+
+/* tmask_<rank>.dat of common/model_mod.f90:251-291 (dump_tmask): raw = 0,
+ * '(2e16.7,1x,I3)' of x, y, tmask per internal T point; raw = 1,
+ * '(3(I4,1x))' of the 1-based ji, jj and tmask over the whole M x N mask.
+ * An empty record after each row. Returns 0 or -1. */
+int nemo_dump_tmask(const char *path, const int *tmask, int width, int M, int N, int xs,
+                    int xe, int ys, int ye, double dx, double dy, int raw);
+
 #ifdef __cplusplus
 }
 #endif

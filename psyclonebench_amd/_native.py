@@ -60,6 +60,13 @@ def setup_lib() -> C.CDLL:
         lib.nemo_field_checksum.argtypes = [_dp] + [C.c_int] * 5
         lib.nemo_field_checksum.restype = C.c_double
         lib.nemo_uniforms.argtypes = [C.c_uint64, C.c_int64, _dp]
+        lib.nemo_fortran_e.argtypes = [C.c_double, C.c_int, C.c_int, C.c_char_p]
+        lib.nemo_write_results.argtypes = [C.c_char_p] + [_dp] * 4 + [C.c_int] * 5 + \
+            [C.c_double] * 2
+        lib.nemo_write_results.restype = C.c_int
+        lib.nemo_dump_tmask.argtypes = [C.c_char_p, _ip] + [C.c_int] * 7 + [C.c_double] * 2 + \
+            [C.c_int]
+        lib.nemo_dump_tmask.restype = C.c_int
         _setup = lib
     return _setup
 

@@ -1,6 +1,7 @@
 """Builds the native libraries of psyclonebench_amd in-tree.
 
-* ``libnemo_setup.so``  host model set-up (g++, OpenMP)
+* ``libnemo_setup.so``  host model set-up and the driver's result / mask
+  files (g++, OpenMP)
 * ``libnemo_hip.so``    HIP kernels + C-ABI PSy layer for gfx950 (hipcc)
 * ``libnemo_hip_cpp.so`` re-exports the host-pointer step under the
   reference's psykal_cpp name ``c_invoke_time_step``
@@ -65,7 +66,8 @@ def _run(cmd: list[str], verbose: bool) -> None:
 def build_setup(verbose: bool = True) -> Path:
     out = PKG / "libnemo_setup.so"
     _run(["g++", "-O2", "-std=c++17", "-fopenmp", "-fPIC", "-shared",
-          f"-I{INCLUDE}", str(CSRC / "nemo_setup.cpp"), "-o", str(out)], verbose)
+          f"-I{INCLUDE}", str(CSRC / "nemo_setup.cpp"), str(CSRC / "nemo_io.cpp"),
+          "-o", str(out)], verbose)
     return out
 
 
