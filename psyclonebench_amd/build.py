@@ -5,6 +5,8 @@
 * ``libnemo_hip.so``    HIP kernels + C-ABI PSy layer for gfx950 (hipcc)
 * ``libnemo_hip_cpp.so`` re-exports the host-pointer step under the
   reference's psykal_cpp name ``c_invoke_time_step``
+* ``nemolite2d_hip``    Fortran driver over the C ABI (amdflang), where a
+  Fortran compiler exists
 
 Every build runs the compilers again (no staleness check): callers rely on
 ``build()`` to pick up source edits.
@@ -15,6 +17,7 @@ import os
 import shutil
 import subprocess
 import sys
+import tempfile
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
@@ -83,9 +86,34 @@ def build_hip(verbose: bool = True) -> Path:
     return out
 
 
+def _fortran() -> str | None:
+    for cand in (os.environ.get("FC"), "/opt/rocm/bin/amdflang", shutil.which("amdflang"),
+                 shutil.which("flang")):
+        if cand and Path(cand).exists():
+            return cand
+    return None
+
+
+def build_fortran(verbose: bool = True) -> Path | None:
+    """``nemolite2d_hip``: the Fortran driver over the C ABI (ROCm's
+    amdflang). Skipped, with a message, where no Fortran compiler exists."""
+    fc = _fortran()
+    out = PKG / "nemolite2d_hip"
+    if fc is None:
+        if verbose:
+            print("no Fortran compiler: nemolite2d_hip not built", flush=True)
+        return None
+    with tempfile.TemporaryDirectory() as mods:
+        _run([fc, "-O2", "-module-dir", mods, str(PKG / "fortran" / "nemolite2d_hip.f90"),
+              "-o", str(out), f"-L{PKG}", "-lnemo_hip", "-lnemo_setup",
+              "-Wl,-rpath,$ORIGIN"], verbose)
+    return out
+
+
 def build(verbose: bool = True) -> None:
     build_setup(verbose)
     build_hip(verbose)
+    build_fortran(verbose)
 
 
 if __name__ == "__main__":
