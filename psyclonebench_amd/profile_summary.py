@@ -43,12 +43,15 @@ def counter_means(db: str, counter: str) -> dict[str, tuple[int, float, float]]:
 def main() -> None:
     p = argparse.ArgumentParser()
     p.add_argument("--trace", required=True)
-    p.add_argument("--fetch", required=True)
-    p.add_argument("--write", required=True)
-    p.add_argument("--report", action="append", required=True,
+    p.add_argument("--fetch", help="FETCH_SIZE pass (with --write and --copy-bytes)")
+    p.add_argument("--write", help="WRITE_SIZE pass")
+    p.add_argument("--report", action="append", default=[],
                    help="LABEL=KERNEL_SUBSTRING=ALG_BYTES_PER_LAUNCH (repeatable)")
-    p.add_argument("--copy-bytes", type=float, required=True,
+    p.add_argument("--copy-bytes", type=float,
                    help="bytes k_copy3 reads (= writes) per launch")
+    p.add_argument("--flops", action="append", default=[],
+                   help="LABEL=KERNEL_SUBSTRING=FLOP_PER_LAUNCH (repeatable): achieved "
+                        "TFLOP/s from the trace's average duration")
     a = p.parse_args()
 
     stats = kernel_stats(a.trace)
@@ -56,6 +59,23 @@ def main() -> None:
     print(f"{'kernel':<40} {'calls':>6} {'avg_us':>10} {'total_ms':>10}")
     for name, calls, total, avg in stats:
         print(f"{_short(name)[:40]:<40} {calls:>6} {avg:>10.2f} {total / 1e3:>10.3f}")
+
+    for rep in a.flops:
+        label, sub, flop = rep.split("=")
+        rows = [(name, avg) for name, _, _, avg in stats if sub in name]
+        if not rows:
+            raise SystemExit(f"no kernel matching {sub!r} in the trace")
+        name, dur_us = rows[0]
+        print()
+        print(f"[{label}] kernel: {_short(name)}")
+        print(f"[{label}] flop_per_launch: {float(flop):.4g}")
+        print(f"[{label}] avg_duration_us (kernel trace): {dur_us:.2f}")
+        print(f"[{label}] achieved_TFLOPs: {float(flop) / (dur_us * 1e-6) / 1e12:.1f}")
+
+    if not a.report:
+        return
+    if not (a.fetch and a.write and a.copy_bytes):
+        raise SystemExit("--report needs --fetch, --write and --copy-bytes")
 
     fetch = counter_means(a.fetch, "FETCH_SIZE")
     write = counter_means(a.write, "WRITE_SIZE")
