@@ -344,14 +344,15 @@ def main(argv=None) -> None:
     t.add_argument("--factor", type=int, default=4)
     t.add_argument("--bathy", default="0:64")
     t.add_argument("--steps", type=int, default=500)
-    t.add_argument("--lr", type=float, default=2e-3)
+    # defaults: the recipe of the shipped weights
+    t.add_argument("--lr", type=float, default=1e-3)
     t.add_argument("--lr-min", type=float, default=1e-4)
     t.add_argument("--seed", type=int, default=0)
-    t.add_argument("--log-every", type=int, default=50)
-    t.add_argument("--rollout", type=int, default=20,
+    t.add_argument("--log-every", type=int, default=100)
+    t.add_argument("--rollout", type=int, default=30,
                    help="longest stretch of steps on the corrected state between resets")
-    t.add_argument("--guard-rel", type=float, default=0.05)
-    t.add_argument("--clip", type=float, default=0.0,
+    t.add_argument("--guard-rel", type=float, default=0.25)
+    t.add_argument("--clip", type=float, default=1.0,
                    help="largest global gradient norm (0: no clipping)")
     t.add_argument("--gate-eps", type=float, default=0.01,
                    help="activity gate width in normalised input units (0: no gate)")

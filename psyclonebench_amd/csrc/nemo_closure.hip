@@ -355,67 +355,55 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 // Output layer (32 -> 3 used channels): the 16 rows of v_mfma_f32_16x16x32_bf16
 // hold (dx, co) = dx * 3 + co, so one instruction covers a whole kernel row
 // of 16 input pixels and 32 channels; the shifted sum over dx goes through a
-// per-wavefront LDS tile. 16 input pixels give 12 outputs; four tiles per
-// wavefront and iteration.
+// per-wavefront LDS tile. 16 input pixels give 12 outputs. One wavefront
+// owns one column tile over OUT_RB output rows: it loads each of the
+// OUT_RB + 4 input rows once and keeps them in registers for the five kernel
+// rows that read them, and it keeps the five weight fragments in registers,
+// so no LDS fill and no workgroup barrier. One item per wavefront (the grid
+// covers every item: out_grid()).
+constexpr int OUT_RB = 8;
+
 __global__ __launch_bounds__(256) void k_conv_out(ConvArgs a)
 {
-    constexpr int ROW = 40;
-    __shared__ __attribute__((aligned(16))) __bf16 wsh[5 * 16 * ROW];
-    __shared__ float dsh[4][4][16][17];  // [wave][tile][dx * 3 + co][pixel]
-    {
-        // 16-byte chunks c8 of row (dy, m); all loads before the LDS stores
-        constexpr int NTOT = 5 * 16 * 4, NQ = (NTOT + 255) / 256;
-        const int tid = (int)threadIdx.x;
-        bf16x8 v[NQ];
-#pragma unroll
-        for (int i = 0; i < NQ; i++) {
-            const int q = tid + 256 * i;
-            const int c8 = q & 3, m = (q >> 2) & 15, dy = q >> 6;
-            const int dx = m / 3, co = m - 3 * dx;
-            v[i] = (q < NTOT && m < 15) ? *reinterpret_cast<const bf16x8 *>(
-                                              a.w + ((size_t)(dy * 5 + dx) * COUT4 + co) * 32 + c8 * 8)
-                                        : bf16x8{};
-        }
-#pragma unroll
-        for (int i = 0; i < NQ; i++) {
-            const int q = tid + 256 * i;
-            const int c8 = q & 3, m = (q >> 2) & 15, dy = q >> 6;
-            if (q < NTOT) *reinterpret_cast<bf16x8 *>(&wsh[(dy * 16 + m) * ROW + c8 * 8]) = v[i];
-        }
-    }
-    __syncthreads();
-
+    __shared__ float dsh[4][4][16][17];  // [wave][row][dx * 3 + co][pixel]
     const int lane = (int)(threadIdx.x & 63), wv = (int)(threadIdx.x >> 6);
     const int n = lane & 15, q4 = lane >> 4, kg = q4 * 8;
-    const int nx = a.nx, Wp = nx + 4, T = (nx + 11) / 12;
-    const size_t PL = plane_elems(nx, a.ny), RS = (size_t)Wp * 16;
-    const Band bd = xcd_band(a.ny * T);
-    const int wib = bd.local * 4 + wv, nwb = bd.nlocal * 4;
-    const size_t kof = (size_t)(kg >> 4) * PL + (kg & 15);
-    float(*d)[16][17] = dsh[wv];
+    const int nx = a.nx, ny = a.ny, Wp = nx + 4, T = (nx + 11) / 12;
+    const int nbands = (ny + OUT_RB - 1) / OUT_RB;
+    const Band bd = xcd_band(nbands * T);
+    const int k = bd.local * 4 + wv;
+    if (k >= bd.count) return;
+    const int it = bd.first + k, band = it / T, t = it - band * T;
+    const int y0 = band * OUT_RB, nr = min(OUT_RB, ny - y0), x0 = min(12 * t, nx - 12);
+    const size_t PL = plane_elems(nx, ny), RS = (size_t)Wp * 16;
 
-    for (int k = 4 * wib; k < bd.count; k += 4 * nwb) {
-        const __bf16 *src[4];
+    // A[m = 3 dx + co][k = ci] of kernel row dy; row 15 is zero
+    bf16x8 w[5];
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int it = bd.first + min(k + j, bd.count - 1);
-            const int y = it / T, x0 = min(12 * (it - y * T), nx - 12);
-            src[j] = a.in + kof + ((size_t)y * Wp + x0 + n) * 16;
-        }
-        bf16x8 b[5][4];
+    for (int dy = 0; dy < 5; dy++) {
+        const int dx = n / 3, co = n - 3 * (n / 3);
+        w[dy] = n < 15 ? *reinterpret_cast<const bf16x8 *>(
+                             a.w + ((size_t)(dy * 5 + dx) * COUT4 + co) * 32 + kg)
+                       : bf16x8{};
+    }
+    // padded input rows y0 .. y0 + nr + 3: 16 pixels, channels kg .. kg + 7
+    const __bf16 *src = a.in + (size_t)(kg >> 4) * PL + (kg & 15) + ((size_t)y0 * Wp + x0 + n) * 16;
+    bf16x8 rows[OUT_RB + 4];
+#pragma unroll
+    for (int r = 0; r < OUT_RB + 4; r++)
+        rows[r] = r < nr + 4 ? *reinterpret_cast<const bf16x8 *>(src + r * RS) : bf16x8{};
+
+    float(*d)[16][17] = dsh[wv];
+    const int nc = min(n, 11);
+#pragma unroll
+    for (int g4 = 0; g4 < OUT_RB / 4; g4++) {
+        f32x4 acc[4] = {};
 #pragma unroll
         for (int dy = 0; dy < 5; dy++)
 #pragma unroll
             for (int j = 0; j < 4; j++)
-                b[dy][j] = *reinterpret_cast<const bf16x8 *>(src[j] + dy * RS);
-        f32x4 acc[4] = {};
-#pragma unroll
-        for (int dy = 0; dy < 5; dy++) {
-            const bf16x8 w = *reinterpret_cast<const bf16x8 *>(wsh + (dy * 16 + n) * ROW + kg);
-#pragma unroll
-            for (int j = 0; j < 4; j++)
-                acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, b[dy][j], acc[j], 0, 0, 0);
-        }
+                acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[dy], rows[4 * g4 + j + dy],
+                                                                 acc[j], 0, 0, 0);
 #pragma unroll
         for (int j = 0; j < 4; j++)
 #pragma unroll
@@ -423,8 +411,7 @@ __global__ __launch_bounds__(256) void k_conv_out(ConvArgs a)
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // lane: tile q4, output pixel n of the tile (n < 12 stored)
-        const int nc = min(n, 11);
+        // lane: output row 4 g4 + q4 of the band, pixel n of the tile
         float v[3];
 #pragma unroll
         for (int co = 0; co < 3; co++) {
@@ -436,19 +423,26 @@ __global__ __launch_bounds__(256) void k_conv_out(ConvArgs a)
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const int it = bd.first + k + q4;
-        if (k + q4 >= bd.count || n >= 12) continue;
-        const int y = it / T, t = it - y * T, x = min(12 * t, nx - 12) + n;
-        if (x < 12 * t) continue;
-        const size_t p = (size_t)y * nx + x;
-        const float g = a.gate ? a.gate[p] : 1.0f;
-        float4 r;
-        r.x = v[0] * g;
-        r.y = v[1] * g;
-        r.z = v[2] * g;
-        r.w = 0.0f;
-        reinterpret_cast<float4 *>(a.outf)[p] = r;
+        const int yy = 4 * g4 + q4, x = x0 + n;
+        if (yy < nr && n < 12 && x >= 12 * t) {
+            const size_t p = (size_t)(y0 + yy) * nx + x;
+            const float gt = a.gate ? a.gate[p] : 1.0f;
+            float4 r;
+            r.x = v[0] * gt;
+            r.y = v[1] * gt;
+            r.z = v[2] * gt;
+            r.w = 0.0f;
+            reinterpret_cast<float4 *>(a.outf)[p] = r;
+        }
     }
+}
+
+// Workgroups of k_conv_out: 4 wavefronts each, every item covered once.
+dim3 out_grid(int nx, int ny)
+{
+    const long items = (long)((nx + 11) / 12) * ((ny + OUT_RB - 1) / OUT_RB);
+    const long per_xcd = (items + 7) / 8;
+    return dim3((unsigned)(8 * ((per_xcd + 3) / 4)));
 }
 
 __device__ __forceinline__ bf16x8 gather8(const __bf16 *p, size_t stride)
@@ -662,6 +656,8 @@ __global__ void k_target(const double *__restrict__ fssh, const double *__restri
 // A wet T cell whose ssh the step does not clamp to the tide (bc_ssh sets
 // every wet cell with an open (-1) 4-neighbour): the closure corrects and
 // learns the ssh of these cells only.
+constexpr int GUARD_PPT = 4;  // guard points per thread and pass
+
 __device__ __forceinline__ bool free_ssh_cell(const int *tm, size_t k, int W)
 {
     return tm[k] > 0 && tm[k - 1] >= 0 && tm[k + 1] >= 0 && tm[k - W] >= 0 && tm[k + W] >= 0;
@@ -778,36 +774,51 @@ __global__ __launch_bounds__(256) void k_guard_partial(const float *__restrict__
                                                        double *__restrict__ partial,
                                                        int nx, int ny)
 {
-    __shared__ double red[3][256];
+    // GUARD_PPT points per thread per pass, all loads of a pass in flight
+    // together; wavefront sums by shuffles, then one LDS step over the 4
+    // wavefronts (a fixed order: the same sums every call).
+    __shared__ double red[3][4];
     double e0 = 0.0, e1 = 0.0, bad = 0.0;
-    for (int p = (int)(blockIdx.x * blockDim.x + threadIdx.x); p < nx * ny;
-         p += (int)(gridDim.x * blockDim.x)) {
-        const int J = p / nx + 1, I = p - (J - 1) * nx + 1;
-        const size_t k = (size_t)J * c.W + I;
-        const float4 q = reinterpret_cast<const float4 *>(pred)[p];
-        const double s = c.ssh[k], u = c.u[k], v = c.v[k], h = c.ht[k];
-        const bool wet = c.tmask[k] > 0;
-        const double s1 =
-            free_ssh_cell(c.tmask, k, c.W) ? s + (double)(q.z * nm.out_scale[2]) : s;
-        const double u1 = (I < nx && wet && c.tmask[k + 1] > 0)
-                              ? u + (double)(q.x * nm.out_scale[0]) : u;
-        const double v1 = (J < ny && wet && c.tmask[k + c.W] > 0)
-                              ? v + (double)(q.y * nm.out_scale[1]) : v;
-        e0 += g * s * s + h * (u * u + v * v);
-        const double t = g * s1 * s1 + h * (u1 * u1 + v1 * v1);
-        if (isfinite(t)) e1 += t;
-        else bad += 1.0;
+    const int npx = nx * ny, stride = (int)(gridDim.x * blockDim.x);
+    for (int p0 = (int)(blockIdx.x * blockDim.x + threadIdx.x); p0 < npx;
+         p0 += GUARD_PPT * stride) {
+#pragma unroll
+        for (int j = 0; j < GUARD_PPT; j++) {
+            const int p = p0 + j * stride;
+            if (p >= npx) break;
+            const int J = p / nx + 1, I = p - (J - 1) * nx + 1;
+            const size_t k = (size_t)J * c.W + I;
+            const float4 q = reinterpret_cast<const float4 *>(pred)[p];
+            const double s = c.ssh[k], u = c.u[k], v = c.v[k], h = c.ht[k];
+            const bool wet = c.tmask[k] > 0;
+            const double s1 =
+                free_ssh_cell(c.tmask, k, c.W) ? s + (double)(q.z * nm.out_scale[2]) : s;
+            const double u1 = (I < nx && wet && c.tmask[k + 1] > 0)
+                                  ? u + (double)(q.x * nm.out_scale[0]) : u;
+            const double v1 = (J < ny && wet && c.tmask[k + c.W] > 0)
+                                  ? v + (double)(q.y * nm.out_scale[1]) : v;
+            e0 += g * s * s + h * (u * u + v * v);
+            const double t = g * s1 * s1 + h * (u1 * u1 + v1 * v1);
+            if (isfinite(t)) e1 += t;
+            else bad += 1.0;
+        }
     }
-    red[0][threadIdx.x] = e0;
-    red[1][threadIdx.x] = e1;
-    red[2][threadIdx.x] = bad;
+    for (int off = 32; off > 0; off >>= 1) {
+        e0 += __shfl_xor(e0, off);
+        e1 += __shfl_xor(e1, off);
+        bad += __shfl_xor(bad, off);
+    }
+    const int lane = (int)(threadIdx.x & 63), wv = (int)(threadIdx.x >> 6);
+    if (lane == 0) {
+        red[0][wv] = e0;
+        red[1][wv] = e1;
+        red[2][wv] = bad;
+    }
     __syncthreads();
-    for (int k = 128; k > 0; k >>= 1) {
-        if ((int)threadIdx.x < k)
-            for (int r = 0; r < 3; r++) red[r][threadIdx.x] += red[r][threadIdx.x + k];
-        __syncthreads();
+    if (threadIdx.x < 3) {
+        const int r = (int)threadIdx.x;
+        partial[blockIdx.x * 3 + r] = (red[r][0] + red[r][1]) + (red[r][2] + red[r][3]);
     }
-    if (threadIdx.x < 3) partial[blockIdx.x * 3 + threadIdx.x] = red[threadIdx.x][0];
 }
 
 // Adds the partial sums in workgroup order (the same sum every call) and
@@ -863,15 +874,15 @@ void launch_conv(hipStream_t s, int cin, int cout, Epilogue epi, const ConvArgs 
 {
     if (a.nx < 32 || a.ny <= 0)
         NEMO_FATAL("closure conv: interior %d x %d (nx must be at least 32)", a.nx, a.ny);
-    // k_conv_sh: 2 workgroups per CU (LDS weights and 256 registers);
-    // k_conv_out: 6 (24 KB of LDS, few registers)
+    // k_conv_sh: persistent, 2 workgroups per CU (LDS weights and 256
+    // registers); k_conv_out: one item per wavefront (out_grid)
     const dim3 grid = conv_grid(2), block(256);
     if (cin == 16 && cout == 32 && epi == EPI_HIDDEN)  // the network input: 4 used channels
         k_conv_sh<0, EPI_HIDDEN><<<grid, block, 0, s>>>(a);
     else if (cin == 32 && cout == 32 && epi == EPI_HIDDEN)
         k_conv_sh<2, EPI_HIDDEN><<<grid, block, 0, s>>>(a);
     else if (cin == 32 && cout == 16 && epi == EPI_OUTPUT)
-        k_conv_out<<<conv_grid(6), block, 0, s>>>(a);
+        k_conv_out<<<out_grid(a.nx, a.ny), block, 0, s>>>(a);
     else if (cin == 16 && cout == 32 && epi == EPI_DX)
         k_conv_sh<1, EPI_DX><<<grid, block, 0, s>>>(a);
     else if (cin == 32 && cout == 32 && epi == EPI_DX)
@@ -991,9 +1002,9 @@ void launch_face_ssh(hipStream_t s, const int *flag, const CoarseState &c, int n
     NEMO_LAUNCH_CHECK();
 }
 
-// about one point per thread on a 1024 x 1024 coarse grid, so that every
-// thread's loads are in flight at once
-int guard_blocks() { return 4096; }
+// GUARD_PPT (4) points per thread cover a 1024 x 1024 coarse grid in one
+// pass of 1024 workgroups
+int guard_blocks() { return 1024; }
 
 void launch_guard(hipStream_t s, const float *pred, const Norm &nm, const CoarseState &c,
                   double g, double *partial, double rel, double abs_tol, int *flag,
