@@ -133,7 +133,7 @@ def mixed_leg(pristine, a, pts, f64_ms_per_step, f64_kernel_ms, f64_variant) -> 
     st = MixedStepper(pristine.copy())
     st.run(1 + a.warmup)
     tune = {}
-    for rows in (16, 32, 64, 128):
+    for rows in (16, 32, 48, 64, 80, 96, 128):
         lib.nemo_mp_set_rows(st.handle, rows)
         tune[rows] = st.timed_run(a.tune_steps)
     rows = min(tune, key=tune.get)

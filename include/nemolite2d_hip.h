@@ -175,7 +175,7 @@ NEMO_API void nemo_mp_steps(void *handle, int nsteps, int istep);
 /* The same steps as plain launches, each timed with HIP events; returns the
  * mean duration in ms of the mixed step kernel. */
 NEMO_API double nemo_mp_kernel_ms(void *handle, int nsteps, int istep);
-/* Rows per band of the kernel: 16, 32 (default), 64 or 128. */
+/* Rows per band of the kernel: 16, 32 (default), 48, 64, 80, 96 or 128. */
 NEMO_API void nemo_mp_set_rows(void *handle, int rows);
 /* Blocking read-back as FP64 (total_size values): field 0 sshn_t (hi+lo),
  * 1 un, 2 vn, 3 sshn_u, 4 sshn_v. */

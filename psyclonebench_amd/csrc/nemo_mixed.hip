@@ -103,7 +103,10 @@ void launch_mixed_step(hipStream_t s, int rows, const MixedIn &f,
     }
         NEMO_MIXED_CASE(16)
         NEMO_MIXED_CASE(32)
+        NEMO_MIXED_CASE(48)
         NEMO_MIXED_CASE(64)
+        NEMO_MIXED_CASE(80)
+        NEMO_MIXED_CASE(96)
         NEMO_MIXED_CASE(128)
 #undef NEMO_MIXED_CASE
     default: NEMO_FATAL("mixed step: unsupported rows per band %d", rows);

@@ -208,8 +208,10 @@ void *nemo_mp_create(NEMO_HOST_ARRAYS, int internal_xstart, int internal_xstop,
 
 void nemo_mp_set_rows(void *h, int rows)
 {
-    if (rows != 16 && rows != 32 && rows != 64 && rows != 128)
-        NEMO_FATAL("mixed form: rows per band must be 16, 32, 64 or 128 (got %d)", rows);
+    if (rows != 16 && rows != 32 && rows != 48 && rows != 64 && rows != 80 && rows != 96 &&
+        rows != 128)
+        NEMO_FATAL("mixed form: rows per band must be 16, 32, 48, 64, 80, 96 or 128 (got %d)",
+                   rows);
     static_cast<MixedContext *>(h)->rows = rows;
 }
 

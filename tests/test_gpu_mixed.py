@@ -75,7 +75,7 @@ def test_mixed_rest_state_stays_zero():
 def test_mixed_band_shapes_bitwise_equal():
     cfg = c2_config(509, 1, seed=3)
     outs = []
-    for rows in (16, 32, 64, 128):
+    for rows in (16, 32, 48, 64, 80, 96, 128):
         st = MixedStepper(Model(cfg), rows=rows)
         st.run(30)
         outs.append({n: st.fetch(n) for n in ("sshn_t", "un", "vn")})
