@@ -274,6 +274,9 @@ NEMO_API double nemo_cl_last_loss(void *handle);
 /* gradients scaled to a global L2 norm of at most max_norm before the
  * optimizer step (0: no clipping) */
 NEMO_API void nemo_cl_set_clip(void *handle, double max_norm);
+/* weights of the u, v and ssh squared errors in the training loss
+ * (default 1, 1, 1); >= 0 */
+NEMO_API void nemo_cl_set_loss_weights(void *handle, double wu, double wv, double wssh);
 /* out[7]: mean squared error of the last loss per channel (u, v, ssh),
  * gradient norm of the last optimizer step, and the energies before / after
  * the last guarded correction and its non-finite count */

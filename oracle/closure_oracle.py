@@ -132,17 +132,18 @@ def forward(params: np.ndarray, x4: np.ndarray, quant: bool = True):
 
 
 def loss_and_grad(params: np.ndarray, x4: np.ndarray, target: np.ndarray,
-                  quant: bool = True) -> tuple[float, np.ndarray]:
-    """Mean squared error over the 3 output channels and its gradient in the
-    parameter layout."""
+                  quant: bool = True, weights=(1.0, 1.0, 1.0)) -> tuple[float, np.ndarray]:
+    """Weighted mean squared error over the 3 output channels,
+    sum_c w_c sum_p d_c^2 / (3 n), and its gradient in the parameter layout."""
     layers = unpack(params)
     out, cache = forward(params, x4, quant)
     ny, nx, _ = out.shape
     n = ny * nx
     diff = out - target
-    loss = float(np.sum(diff * diff) / (3 * n))
+    w = np.asarray(weights, np.float64)
+    loss = float(np.sum(w * diff * diff) / (3 * n))
     d = np.zeros((ny, nx, LAYERS[3][1]))
-    d[..., :USED_OUT] = 2.0 * diff / (3 * n)
+    d[..., :USED_OUT] = 2.0 * w * diff / (3 * n)
     d = _q(d, quant)
     grads = np.zeros(param_count())
     offs = layer_offsets()

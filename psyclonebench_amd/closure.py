@@ -170,6 +170,9 @@ class ClosureNet:
     def set_clip(self, max_norm: float) -> None:
         hip_lib().nemo_cl_set_clip(self.handle, max_norm)
 
+    def set_loss_weights(self, wu: float, wv: float, wssh: float) -> None:
+        hip_lib().nemo_cl_set_loss_weights(self.handle, wu, wv, wssh)
+
     def last_stats(self) -> dict:
         o = np.zeros(7, np.float64)
         hip_lib().nemo_cl_last_stats(self.handle, o.ctypes.data_as(C.POINTER(C.c_double)))

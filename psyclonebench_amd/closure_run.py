@@ -172,6 +172,7 @@ def train(a) -> None:
             net = ClosureNet(coarse, params=params)
             configure(net, norm)
             net.set_clip(a.clip)
+            net.set_loss_weights(*a.loss_weights)
         g = coarse.grid
         net.set_guard(a.guard_rel, guard_abs(coarse))
         net.filter_fine(sf, a.factor)
@@ -215,7 +216,7 @@ def save(path, params, norm, a, history) -> None:
     meta = {"norm": norm, "n_fine": a.n, "factor": a.factor, "bathy": a.bathy,
             "coarse_steps_per_seed": a.steps, "lr": a.lr, "lr_min": a.lr_min,
             "rollout": a.rollout, "guard_rel": a.guard_rel, "clip": a.clip,
-            "max_spin": a.max_spin,
+            "max_spin": a.max_spin, "loss_weights": list(a.loss_weights),
             "optimizer": "adam", "history": history}
     Path(path).parent.mkdir(parents=True, exist_ok=True)
     np.savez(path, params=np.asarray(params, np.float32), meta=json.dumps(meta))
@@ -354,6 +355,8 @@ def main(argv=None) -> None:
     t.add_argument("--guard-rel", type=float, default=0.25)
     t.add_argument("--clip", type=float, default=1.0,
                    help="largest global gradient norm (0: no clipping)")
+    t.add_argument("--loss-weights", type=lambda s: tuple(float(x) for x in s.split(",")),
+                   default=(1.0, 1.0, 1.0), help="u,v,ssh weights of the squared errors")
     t.add_argument("--gate-eps", type=float, default=0.01,
                    help="activity gate width in normalised input units (0: no gate)")
     t.add_argument("--pilot-steps", type=int, default=2000,

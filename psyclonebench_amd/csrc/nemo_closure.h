@@ -93,13 +93,19 @@ void launch_pack(hipStream_t s, const CoarseState &c, const Norm &nm, __bf16 *in
 // target[(y*nx+x)*4 + k] = (F_k - C_k) / out_scale_k, k = u, v, ssh
 void launch_target(hipStream_t s, const double *fssh, const double *fu, const double *fv,
                    const CoarseState &c, const Norm &nm, float *target, int nx, int ny);
-// MSE loss over the 3 used channels: gradient into the padded NHWC dpre4
-// (COUT4 channels) and per-block partial sums of the squared error. With a
-// mask (pitch W), the ssh of cells the step clamps to the tide is left out.
+// Per-channel weights (u, v, ssh) of the squared error in the loss.
+struct LossWeights {
+    float w[3];
+};
+
+// Weighted MSE loss over the 3 used channels, sum_c w_c sum_p d_c^2 / (3 npx):
+// gradient into dpre4 (COUT4 channels, 16-channel plane) and per-block
+// partial sums of the unweighted squared error per channel. With a mask
+// (pitch W), the ssh of cells the step clamps to the tide is left out.
 int loss_blocks();
 void launch_loss(hipStream_t s, const float *pred, const float *target, __bf16 *dpre4,
                  float *partial, int nx, int ny, const int *tmask, int W,
-                 const float *gate);
+                 const float *gate, const LossWeights &lw);
 
 // ---- fine <-> coarse plumbing ----------------------------------------------
 // Box filter of the fine state onto the coarse grid (factor f): T cells by

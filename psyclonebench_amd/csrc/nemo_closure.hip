@@ -668,7 +668,7 @@ __global__ __launch_bounds__(256) void k_loss(const float *__restrict__ pred,
                                               __bf16 *__restrict__ dpre4,
                                               float *__restrict__ partial, int nx, int ny,
                                               const int *__restrict__ tmask, int W,
-                                              const float *__restrict__ gate)
+                                              const float *__restrict__ gate, LossWeights lw)
 {
     __shared__ float red[3][256];
     const int npx = nx * ny;
@@ -688,9 +688,9 @@ __global__ __launch_bounds__(256) void k_loss(const float *__restrict__ pred,
         // the output is gated (pred = gate * raw output): d loss / d raw
         const float gs = gate ? scale * gate[p] : scale;
         bf16x8 lo = {}, hi = {};
-        lo[0] = (__bf16)(gs * d0);
-        lo[1] = (__bf16)(gs * d1);
-        lo[2] = (__bf16)(gs * d2);
+        lo[0] = (__bf16)(gs * lw.w[0] * d0);
+        lo[1] = (__bf16)(gs * lw.w[1] * d1);
+        lo[2] = (__bf16)(gs * lw.w[2] * d2);
         bf16x8 *dst = reinterpret_cast<bf16x8 *>(dpre4 + pad_index(y, x, nx) * COUT4);
         dst[0] = lo;
         dst[1] = hi;
@@ -970,11 +970,11 @@ int loss_blocks() { return 256; }
 
 void launch_loss(hipStream_t s, const float *pred, const float *target, __bf16 *dpre4,
                  float *partial, int nx, int ny, const int *tmask, int W,
-                 const float *gate)
+                 const float *gate, const LossWeights &lw)
 {
     k_loss<<<dim3((unsigned)loss_blocks()), dim3(256), 0, s>>>(pred, target, dpre4,
                                                                partial, nx, ny, tmask, W,
-                                                               gate);
+                                                               gate, lw);
     NEMO_LAUNCH_CHECK();
 }
 

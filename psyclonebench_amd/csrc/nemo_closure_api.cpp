@@ -33,6 +33,7 @@ struct ClosureContext {
     int *flag = nullptr, *trips = nullptr;
     int opt_kind = 1;
     float lr = 1e-3f, b1 = 0.9f, b2 = 0.999f, eps = 1e-8f, max_norm = 0.0f;
+    LossWeights loss_w{{1.0f, 1.0f, 1.0f}};
     float *norm_scale = nullptr;  // device: gradient norm, clip factor
     long long step = 0;
     double guard_rel = 0.05, guard_abs = 0.0, g = 9.80665;
@@ -348,7 +349,7 @@ void nemo_cl_train_step(void *h, double *ssh, double *u, double *v, double *sshu
     forward(C, s, true, true);
     launch_target(s, C.fssh, C.fu, C.fv, c, C.nm, C.target, C.nx, C.ny);
     launch_loss(s, C.pred, C.target, C.dpre[NLAYER - 1], C.loss_partial, C.nx, C.ny, tmask,
-                C.Wc, C.gate);
+                C.Wc, C.gate, C.loss_w);
     backward(C, s);
     optimizer_step(C, s);
 }
@@ -366,6 +367,13 @@ double nemo_cl_last_loss(void *h)
 }
 
 void nemo_cl_set_clip(void *h, double max_norm) { ctx(h).max_norm = (float)max_norm; }
+
+void nemo_cl_set_loss_weights(void *h, double wu, double wv, double wssh)
+{
+    if (!(wu >= 0.0 && wv >= 0.0 && wssh >= 0.0))
+        NEMO_FATAL("closure: loss weights must be >= 0 (got %g, %g, %g)", wu, wv, wssh);
+    ctx(h).loss_w = LossWeights{{(float)wu, (float)wv, (float)wssh}};
+}
 
 void nemo_cl_last_stats(void *h, double *out)
 {
@@ -457,7 +465,7 @@ static void host_pair(ClosureContext &C, const float *in4, const float *target3,
                                   hipMemcpyHostToDevice, s));
     forward(C, s, true, false);
     launch_loss(s, C.pred, C.target, C.dpre[NLAYER - 1], C.loss_partial, C.nx, C.ny,
-                nullptr, 0, nullptr);
+                nullptr, 0, nullptr, C.loss_w);
     backward(C, s);
     NEMO_HIP_CHECK(hipStreamSynchronize(s));  // t goes out of scope
 }

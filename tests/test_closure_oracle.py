@@ -89,7 +89,8 @@ def test_analytic_gradient_matches_finite_differences():
     p[offs[3]:offs[4]] *= 10.0
     x = rng.standard_normal((6, 8, 4))
     t = 0.3 * rng.standard_normal((6, 8, 3))
-    _, g = co.loss_and_grad(p, x, t, quant=False)
+    w = (1.0, 3.0, 0.5)   # channel weights: the weighted loss and its gradient
+    _, g = co.loss_and_grad(p, x, t, quant=False, weights=w)
     picks = []
     for l, (cin, cout) in enumerate(co.LAYERS):
         ci_used = 4 if l == 0 else cin
@@ -102,8 +103,8 @@ def test_analytic_gradient_matches_finite_differences():
     for k in picks:
         e = np.zeros_like(p)
         e[k] = h
-        lp, _ = co.loss_and_grad(p + e, x, t, quant=False)
-        lm, _ = co.loss_and_grad(p - e, x, t, quant=False)
+        lp, _ = co.loss_and_grad(p + e, x, t, quant=False, weights=w)
+        lm, _ = co.loss_and_grad(p - e, x, t, quant=False, weights=w)
         fd = (lp - lm) / (2 * h)
         assert np.isclose(g[k], fd, rtol=2e-5, atol=1e-10), (k, g[k], fd)
 

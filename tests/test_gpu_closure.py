@@ -70,17 +70,21 @@ def test_forward_matches_oracle():
     net.release()
 
 
-def test_gradients_match_oracle():
+@pytest.mark.parametrize("weights", [(1.0, 1.0, 1.0), (1.0, 3.0, 0.5)])
+def test_gradients_match_oracle(weights):
     m = coarse_test_model(64, 16)
     p = net_params(4)
     net = ClosureNet(m, params=p)
+    net.set_loss_weights(*weights)
     rng = np.random.default_rng(1)
     x = rng.standard_normal((16, 64, 4))
     t = rng.standard_normal((16, 64, 3))
     loss_g, g = net.grad_host(x, t)
-    loss_o, go = co.loss_and_grad(p.astype(np.float64), x, t, quant=True)
-    print("loss gpu / oracle:", loss_g, loss_o, flush=True)
-    assert abs(loss_g - loss_o) / loss_o < FWD_TOL
+    loss_o, go = co.loss_and_grad(p.astype(np.float64), x, t, quant=True, weights=weights)
+    # the reported loss is the unweighted mean squared error
+    loss_u, _ = co.loss_and_grad(p.astype(np.float64), x, t, quant=True)
+    print("weights", weights, "loss gpu / oracle:", loss_g, loss_u, flush=True)
+    assert abs(loss_g - loss_u) / loss_u < FWD_TOL
     offs = layer_offsets()
     for l, (cin, cout) in enumerate(co.LAYERS):
         nw = 25 * cin * cout
