@@ -356,7 +356,7 @@ def main(argv=None) -> None:
     t.add_argument("--clip", type=float, default=1.0,
                    help="largest global gradient norm (0: no clipping)")
     t.add_argument("--loss-weights", type=lambda s: tuple(float(x) for x in s.split(",")),
-                   default=(1.0, 1.0, 1.0), help="u,v,ssh weights of the squared errors")
+                   default=(1.0, 3.0, 1.0), help="u,v,ssh weights of the squared errors")
     t.add_argument("--gate-eps", type=float, default=0.01,
                    help="activity gate width in normalised input units (0: no gate)")
     t.add_argument("--pilot-steps", type=int, default=2000,
