@@ -223,8 +223,8 @@ NEMO_API void nemo_ens_destroy(void *handle);
 
 /* ---- learned coarse-grid closure (no reference counterpart) -----------
  * A 4-layer 5x5 convolutional network (4 -> 32 -> 32 -> 32 -> 3 channels,
- * GELU) on the coarse grid's interior T points (nx x ny, nx a multiple of
- * 32), bf16 MFMA with FP32 accumulation; inputs u, v, ssh, depth, outputs
+ * GELU in tanh form) on the coarse grid's interior T points (nx x ny, nx a
+ * multiple of 32), bf16 MFMA with FP32 accumulation; inputs u, v, ssh, depth, outputs
  * corrections to u, v, ssh. Parameters are FP32, nemo_cl_param_count() of
  * them, layer by layer: weights [25 taps][cout][cin] (cin 16 for layer 0 of
  * which 4 are used, cout 16 for layer 3 of which 3 are used) then bias.
