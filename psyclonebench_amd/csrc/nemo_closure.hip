@@ -522,26 +522,37 @@ __global__ __launch_bounds__(256) void k_dw(const __bf16 *__restrict__ dpre,
     }
 }
 
+// Partial bias gradient: per channel, the sum over this workgroup's pixels of
+// dPre. A thread reads one 16-byte chunk (8 channels of one pixel) per
+// iteration, CO / 8 chunks a pixel, so a wavefront's loads cover contiguous
+// pixels of each plane; the sums of the threads that share a chunk are added
+// in LDS in a fixed order.
 template <int CO>
 __global__ __launch_bounds__(256) void k_bias_partial(const __bf16 *__restrict__ dpre,
                                                       float *__restrict__ partial, int nx,
                                                       int ny)
 {
-    constexpr int PL = 256 / CO;  // pixel lanes
-    __shared__ float red[256];
-    const int c = (int)threadIdx.x % CO, q = (int)threadIdx.x / CO;
+    constexpr int NCH = CO / 8;     // chunks per pixel
+    constexpr int PPI = 256 / NCH;  // pixels per workgroup and iteration
+    __shared__ float red[256][9];
+    const int ch = (int)threadIdx.x % NCH, q = (int)threadIdx.x / NCH;
+    const size_t coff = (size_t)(ch >> 1) * plane_elems(nx, ny) + (size_t)(ch & 1) * 8;
     const int npx = nx * ny;
-    float s = 0.0f;
-    for (int p = (int)blockIdx.x * PL + q; p < npx; p += (int)gridDim.x * PL) {
+    float s[8] = {};
+#pragma unroll 4
+    for (int p = (int)blockIdx.x * PPI + q; p < npx; p += (int)gridDim.x * PPI) {
         const int y = p / nx, x = p - y * nx;
-        s += (float)dpre[(size_t)(c >> 4) * plane_elems(nx, ny) + pad_index(y, x, nx) * 16 +
-                         (c & 15)];
+        const bf16x8 v = *reinterpret_cast<const bf16x8 *>(dpre + coff + pad_index(y, x, nx) * 16);
+#pragma unroll
+        for (int k = 0; k < 8; k++) s[k] += (float)v[k];
     }
-    red[threadIdx.x] = s;
+#pragma unroll
+    for (int k = 0; k < 8; k++) red[threadIdx.x][k] = s[k];
     __syncthreads();
-    if (q == 0) {
+    if ((int)threadIdx.x < CO) {
+        const int c = (int)threadIdx.x, chunk = c >> 3, k = c & 7;
         float t = 0.0f;
-        for (int k = 0; k < PL; k++) t += red[k * CO + c];
+        for (int qq = 0; qq < PPI; qq++) t += red[qq * NCH + chunk][k];
         partial[(size_t)blockIdx.x * CO + c] = t;
     }
 }
