@@ -68,8 +68,9 @@ struct ConvArgs {
 };
 
 void launch_conv(hipStream_t s, int cin, int cout, Epilogue epi, const ConvArgs &a);
-// dW partial sums: partial[block][TAPS][co][ci] over the pixels of each block
-int dw_blocks();
+// dW partial sums: partial[block][TAPS][co][ci] over the pixels of each of
+// the dw_blocks(nx, ny) workgroups
+int dw_blocks(int nx, int ny);
 void launch_dw(hipStream_t s, int co, int ci, const __bf16 *dpre, const __bf16 *in,
                float *partial, int nx, int ny);
 int bias_blocks();

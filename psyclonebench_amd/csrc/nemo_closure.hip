@@ -445,28 +445,96 @@ dim3 out_grid(int nx, int ny)
     return dim3((unsigned)(8 * ((per_xcd + 3) / 4)));
 }
 
-__device__ __forceinline__ bf16x8 gather8(const __bf16 *p, size_t stride)
-{
-    bf16x8 r;
-#pragma unroll
-    for (int j = 0; j < 8; j++) r[j] = p[j * stride];
-    return r;
-}
+// Weight gradient, dW[tap][co][ci] = sum over pixels of dPre[px][co] *
+// X[px + tap][ci]: a GEMM whose K runs over pixels, on
+// v_mfma_f32_16x16x32_bf16 (16 co x 16 ci x 32 pixels). Its operands want 8
+// consecutive pixels of one channel per lane, while the activations store
+// channels fastest, so a workgroup stages them in LDS transposed ([channel]
+// [pixel]) as it marches down a 32-pixel column strip: per output row one
+// dPre row and one new input row, the input rows in a ring of the 5 kernel
+// rows. Each input row is staged at 4 pixel shifts so that the fragment of
+// every horizontal tap is an aligned read (dx 0..3: 16 bytes from shift dx;
+// dx 4: two 8-byte reads from shift 0). Wave w accumulates the taps w, w+4,
+// ...; the next row's global loads are in flight during the current row's
+// MFMAs. Every workgroup writes its partial dW; k_reduce adds the partials
+// in a fixed order. CI == CIN0 is layer 0: its input is the IN0C-channel
+// network input, and staged channels IN0C..15 stay zero.
+constexpr int DW_LEN = 40;  // staged pixels per row and shift (36 used)
 
-// Partial weight gradient of one workgroup. Wave w handles the taps w,
-// w+4, ...; the dPre fragment of a 32-pixel chunk is shared by its taps.
-// CI == CIN0 is layer 0, whose input is the IN0C-channel network input.
 template <int CO, int CI>
 __global__ __launch_bounds__(256) void k_dw(const __bf16 *__restrict__ dpre,
                                             const __bf16 *__restrict__ in,
-                                            float *__restrict__ partial, int nx, int ny)
+                                            float *__restrict__ partial, int nx, int ny,
+                                            int rows_per_block)
 {
     constexpr int MB = CO / 16, NB = CI / 16, MAXT = 7;
-    const int lane = (int)(threadIdx.x & 63);
-    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int pl = lane & 15, kg = (lane >> 4) * 8;
-    const int Wp = nx + 4, ntx = nx >> 5, items = ny * ntx;
+    constexpr bool NET_IN = CI == CIN0;
+    constexpr int XQ = NET_IN ? 36 : 36 * (CI / 8);  // 16-byte (8-byte) chunks of an input row
+    constexpr int DQ = 32 * (CO / 8);                // 16-byte chunks of a dPre row
+    static_assert(XQ <= 256 && DQ <= 256, "one chunk per thread");
+    __shared__ __attribute__((aligned(16))) __bf16 xs[5][4][CI][DW_LEN];
+    __shared__ __attribute__((aligned(16))) __bf16 dsh[CO][32];
+    const int tid = (int)threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int n16 = lane & 15, kg = (lane >> 4) * 8;
+    const int Wp = nx + 4, nstrip = nx >> 5;
     const size_t PL = plane_elems(nx, ny);
+    const int strip = (int)blockIdx.x % nstrip, chunk = (int)blockIdx.x / nstrip;
+    const int x0 = strip * 32, ya = chunk * rows_per_block;
+    const int yb = min(ya + rows_per_block, ny);
+
+    if constexpr (NET_IN) {
+        // channels IN0C..15 are never staged: zero them once
+        bf16x8 *z = reinterpret_cast<bf16x8 *>(&xs[0][0][0][0]);
+        for (int q = tid; q < (int)(sizeof(xs) / 16); q += 256) z[q] = bf16x8{};
+        __syncthreads();
+    }
+
+    bf16x8 xreg = {}, dreg = {};
+    auto load_x = [&](int r) __attribute__((always_inline)) {  // padded input row r
+        if (tid < XQ) {
+            if constexpr (NET_IN) {
+                const bf16x4 v =
+                    *reinterpret_cast<const bf16x4 *>(in + ((size_t)r * Wp + x0 + tid) * IN0C);
+                xreg = __builtin_shufflevector(v, bf16x4{}, 0, 1, 2, 3, 4, 5, 6, 7);
+            } else {
+                const int p = tid / (CI / 8), c8 = tid - p * (CI / 8);
+                xreg = *reinterpret_cast<const bf16x8 *>(
+                    in + (size_t)(c8 >> 1) * PL + ((size_t)r * Wp + x0 + p) * 16 + (c8 & 1) * 8);
+            }
+        }
+    };
+    auto store_x = [&](int r) __attribute__((always_inline)) {
+        if (tid < XQ) {
+            const int slot = r % 5;
+            const int p = NET_IN ? tid : tid / (CI / 8);
+            const int cb = NET_IN ? 0 : (tid - p * (CI / 8)) * 8;
+            constexpr int NCH = NET_IN ? IN0C : 8;
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                const int j = p - s;
+                if (j >= 0) {
+#pragma unroll
+                    for (int k = 0; k < NCH; k++) xs[slot][s][cb + k][j] = xreg[k];
+                }
+            }
+        }
+    };
+    auto load_d = [&](int y) __attribute__((always_inline)) {  // interior row y
+        if (tid < DQ) {
+            const int p = tid / (CO / 8), c8 = tid - p * (CO / 8);
+            dreg = *reinterpret_cast<const bf16x8 *>(
+                dpre + (size_t)(c8 >> 1) * PL + pad_index(y, x0 + p, nx) * 16 + (c8 & 1) * 8);
+        }
+    };
+    auto store_d = [&]() __attribute__((always_inline)) {
+        if (tid < DQ) {
+            const int p = tid / (CO / 8), cb = (tid - p * (CO / 8)) * 8;
+#pragma unroll
+            for (int k = 0; k < 8; k++) dsh[cb + k][p] = dreg[k];
+        }
+    };
+
     f32x4 acc[MAXT][MB][NB];
 #pragma unroll
     for (int t = 0; t < MAXT; t++)
@@ -475,26 +543,42 @@ __global__ __launch_bounds__(256) void k_dw(const __bf16 *__restrict__ dpre,
 #pragma unroll
             for (int nb = 0; nb < NB; nb++) acc[t][mb][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    for (int it = (int)blockIdx.x; it < items; it += (int)gridDim.x) {
-        const int y = it / ntx, x0 = (it - y * ntx) * 32;
+    // the ring holds padded input rows y .. y+4 while output row y is reduced
+    for (int r = ya; r < ya + 4; r++) {
+        load_x(r);
+        store_x(r);
+    }
+    load_x(ya + 4);
+    load_d(ya);
+    for (int y = ya; y < yb; y++) {
+        store_x(y + 4);
+        store_d();
+        __syncthreads();
+        if (y + 1 < yb) {
+            load_x(y + 5);
+            load_d(y + 1);
+        }
         bf16x8 A[MB];
-        const __bf16 *dp = dpre + ((size_t)(y + 2) * Wp + x0 + kg + 2) * 16 + pl;
 #pragma unroll
-        for (int mb = 0; mb < MB; mb++) A[mb] = gather8(dp + mb * PL, 16);
+        for (int mb = 0; mb < MB; mb++)
+            A[mb] = *reinterpret_cast<const bf16x8 *>(&dsh[mb * 16 + n16][kg]);
 #pragma unroll
         for (int t = 0; t < MAXT; t++) {
             const int tap = wv + 4 * t;
-            if (tap < TAPS) {
+            if (tap < TAPS) {  // wavefront-uniform
                 const int dy = tap / 5, dx = tap - 5 * (tap / 5);
-                const size_t pix = (size_t)(y + dy) * Wp + x0 + kg + dx;
+                const int slot = (y + dy) % 5;
 #pragma unroll
                 for (int nb = 0; nb < NB; nb++) {
                     bf16x8 B;
-                    if constexpr (CI == CIN0) {
-                        // the network input: IN0C channels a pixel, the others zero
-                        B = pl < IN0C ? gather8(in + pix * IN0C + pl, IN0C) : bf16x8{};
+                    if (dx < 4) {
+                        B = *reinterpret_cast<const bf16x8 *>(&xs[slot][dx][nb * 16 + n16][kg]);
                     } else {
-                        B = gather8(in + pix * 16 + pl + nb * PL, 16);
+                        const bf16x4 lo =
+                            *reinterpret_cast<const bf16x4 *>(&xs[slot][0][nb * 16 + n16][4 + kg]);
+                        const bf16x4 hi =
+                            *reinterpret_cast<const bf16x4 *>(&xs[slot][0][nb * 16 + n16][8 + kg]);
+                        B = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
                     }
 #pragma unroll
                     for (int mb = 0; mb < MB; mb++)
@@ -503,6 +587,7 @@ __global__ __launch_bounds__(256) void k_dw(const __bf16 *__restrict__ dpre,
                 }
             }
         }
+        __syncthreads();
     }
     float *out = partial + (size_t)blockIdx.x * TAPS * CO * CI;
 #pragma unroll
@@ -516,7 +601,7 @@ __global__ __launch_bounds__(256) void k_dw(const __bf16 *__restrict__ dpre,
 #pragma unroll
                     for (int r = 0; r < 4; r++) {
                         const int co = mb * 16 + (lane >> 4) * 4 + r;
-                        out[((size_t)tap * CO + co) * CI + nb * 16 + pl] = acc[t][mb][nb][r];
+                        out[((size_t)tap * CO + co) * CI + nb * 16 + n16] = acc[t][mb][nb][r];
                     }
         }
     }
@@ -904,19 +989,33 @@ void launch_conv(hipStream_t s, int cin, int cout, Epilogue epi, const ConvArgs 
     NEMO_LAUNCH_CHECK();
 }
 
-int dw_blocks() { return 256; }
+// dW workgroups: nx / 32 column strips x row chunks, about 256 in all (at
+// least 8 rows a chunk, so the 4-row ring fill stays a small overhead)
+static int dw_rows_per_block(int nx, int ny)
+{
+    const long nstrip = nx / 32;
+    return (int)std::max(8L, (ny * nstrip + 255) / 256);
+}
+
+int dw_blocks(int nx, int ny)
+{
+    const int rpb = dw_rows_per_block(nx, ny);
+    return (nx / 32) * ((ny + rpb - 1) / rpb);
+}
 
 void launch_dw(hipStream_t s, int co, int ci, const __bf16 *dpre, const __bf16 *in,
                float *partial, int nx, int ny)
 {
-    if (nx % 32 != 0) NEMO_FATAL("closure dW: nx = %d not a multiple of 32", nx);
-    const dim3 grid((unsigned)dw_blocks()), block(256);
+    if (nx % 32 != 0 || nx <= 0 || ny <= 0)
+        NEMO_FATAL("closure dW: interior %d x %d (nx a positive multiple of 32)", nx, ny);
+    const int rpb = dw_rows_per_block(nx, ny);
+    const dim3 grid((unsigned)dw_blocks(nx, ny)), block(256);
     if (co == 32 && ci == 32)
-        k_dw<32, 32><<<grid, block, 0, s>>>(dpre, in, partial, nx, ny);
+        k_dw<32, 32><<<grid, block, 0, s>>>(dpre, in, partial, nx, ny, rpb);
     else if (co == 32 && ci == 16)
-        k_dw<32, 16><<<grid, block, 0, s>>>(dpre, in, partial, nx, ny);
+        k_dw<32, 16><<<grid, block, 0, s>>>(dpre, in, partial, nx, ny, rpb);
     else if (co == 16 && ci == 32)
-        k_dw<16, 32><<<grid, block, 0, s>>>(dpre, in, partial, nx, ny);
+        k_dw<16, 32><<<grid, block, 0, s>>>(dpre, in, partial, nx, ny, rpb);
     else
         NEMO_FATAL("closure dW: unsupported %d x %d", co, ci);
     NEMO_LAUNCH_CHECK();

@@ -127,7 +127,8 @@ void backward(ClosureContext &C, hipStream_t s)
         const int ci = layer_cin(l), co = layer_cout(l);
         const __bf16 *x = l == 0 ? C.in0 : C.act[l - 1];
         launch_dw(s, co, ci, C.dpre[l], x, C.dw_partial, C.nx, C.ny);
-        launch_reduce(s, C.dw_partial, dw_blocks(), layer_wsize(l), C.grads + layer_offset(l));
+        launch_reduce(s, C.dw_partial, dw_blocks(C.nx, C.ny), layer_wsize(l),
+                      C.grads + layer_offset(l));
         launch_bias_partial(s, co, C.dpre[l], C.b_partial, C.nx, C.ny);
         launch_reduce(s, C.b_partial, bias_blocks(), co,
                       C.grads + layer_offset(l) + layer_wsize(l));
@@ -230,7 +231,7 @@ void *nemo_cl_create(int nx, int ny, int coarse_width, int coarse_height,
     C->pred = zeros<float>(npx * 4, s);
     C->gate = zeros<float>(npx, s);
     C->target = zeros<float>(npx * 4, s);
-    C->dw_partial = zeros<float>((size_t)dw_blocks() * TAPS * HID * HID, s);
+    C->dw_partial = zeros<float>((size_t)dw_blocks(nx, ny) * TAPS * HID * HID, s);
     C->b_partial = zeros<float>((size_t)bias_blocks() * HID, s);
     C->loss_partial = zeros<float>((size_t)loss_blocks() * 3, s);
     C->norm_scale = zeros<float>(2, s);
