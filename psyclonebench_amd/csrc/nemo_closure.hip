@@ -13,8 +13,9 @@
 //
 // Weight gradient: dW[tap][co][ci] = sum_px dPre[px][co] * X[px + tap][ci],
 // a GEMM whose reduction runs over pixels, on v_mfma_f32_16x16x32_bf16 with
-// fragments gathered pixel by pixel; every workgroup sums its pixels into a
-// partial dW, and a second pass adds the partials in a fixed order.
+// its operands staged pixel-fastest in LDS (k_dw); every workgroup sums its
+// pixels into a partial dW, and a second pass adds the partials in a fixed
+// order.
 #include <cmath>
 #include <type_traits>
 
@@ -455,11 +456,12 @@ dim3 out_grid(int nx, int ny)
 // rows. Each input row is staged at 4 pixel shifts so that the fragment of
 // every horizontal tap is an aligned read (dx 0..3: 16 bytes from shift dx;
 // dx 4: two 8-byte reads from shift 0). Wave w accumulates the taps w, w+4,
-// ...; the next row's global loads are in flight during the current row's
-// MFMAs. Every workgroup writes its partial dW; k_reduce adds the partials
+// ...; the global loads of the row DW_PF rows ahead are in flight during
+// the current row's MFMAs. Every workgroup writes its partial dW; k_reduce adds the partials
 // in a fixed order. CI == CIN0 is layer 0: its input is the IN0C-channel
 // network input, and staged channels IN0C..15 stay zero.
 constexpr int DW_LEN = 40;  // staged pixels per row and shift (36 used)
+constexpr int DW_PF = 4;    // rows whose global loads are in flight
 
 template <int CO, int CI>
 __global__ __launch_bounds__(256) void k_dw(const __bf16 *__restrict__ dpre,
@@ -490,21 +492,20 @@ __global__ __launch_bounds__(256) void k_dw(const __bf16 *__restrict__ dpre,
         __syncthreads();
     }
 
-    bf16x8 xreg = {}, dreg = {};
-    auto load_x = [&](int r) __attribute__((always_inline)) {  // padded input row r
+    auto load_x = [&](int r, bf16x8 &xr) __attribute__((always_inline)) {  // padded input row r
         if (tid < XQ) {
             if constexpr (NET_IN) {
                 const bf16x4 v =
                     *reinterpret_cast<const bf16x4 *>(in + ((size_t)r * Wp + x0 + tid) * IN0C);
-                xreg = __builtin_shufflevector(v, bf16x4{}, 0, 1, 2, 3, 4, 5, 6, 7);
+                xr = __builtin_shufflevector(v, bf16x4{}, 0, 1, 2, 3, 4, 5, 6, 7);
             } else {
                 const int p = tid / (CI / 8), c8 = tid - p * (CI / 8);
-                xreg = *reinterpret_cast<const bf16x8 *>(
+                xr = *reinterpret_cast<const bf16x8 *>(
                     in + (size_t)(c8 >> 1) * PL + ((size_t)r * Wp + x0 + p) * 16 + (c8 & 1) * 8);
             }
         }
     };
-    auto store_x = [&](int r) __attribute__((always_inline)) {
+    auto store_x = [&](int r, const bf16x8 &xr) __attribute__((always_inline)) {
         if (tid < XQ) {
             const int slot = r % 5;
             const int p = NET_IN ? tid : tid / (CI / 8);
@@ -515,23 +516,23 @@ __global__ __launch_bounds__(256) void k_dw(const __bf16 *__restrict__ dpre,
                 const int j = p - s;
                 if (j >= 0) {
 #pragma unroll
-                    for (int k = 0; k < NCH; k++) xs[slot][s][cb + k][j] = xreg[k];
+                    for (int k = 0; k < NCH; k++) xs[slot][s][cb + k][j] = xr[k];
                 }
             }
         }
     };
-    auto load_d = [&](int y) __attribute__((always_inline)) {  // interior row y
+    auto load_d = [&](int y, bf16x8 &dr) __attribute__((always_inline)) {  // interior row y
         if (tid < DQ) {
             const int p = tid / (CO / 8), c8 = tid - p * (CO / 8);
-            dreg = *reinterpret_cast<const bf16x8 *>(
+            dr = *reinterpret_cast<const bf16x8 *>(
                 dpre + (size_t)(c8 >> 1) * PL + pad_index(y, x0 + p, nx) * 16 + (c8 & 1) * 8);
         }
     };
-    auto store_d = [&]() __attribute__((always_inline)) {
+    auto store_d = [&](const bf16x8 &dr) __attribute__((always_inline)) {
         if (tid < DQ) {
             const int p = tid / (CO / 8), cb = (tid - p * (CO / 8)) * 8;
 #pragma unroll
-            for (int k = 0; k < 8; k++) dsh[cb + k][p] = dreg[k];
+            for (int k = 0; k < 8; k++) dsh[cb + k][p] = dr[k];
         }
     };
 
@@ -543,51 +544,68 @@ __global__ __launch_bounds__(256) void k_dw(const __bf16 *__restrict__ dpre,
 #pragma unroll
             for (int nb = 0; nb < NB; nb++) acc[t][mb][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    // the ring holds padded input rows y .. y+4 while output row y is reduced
+    // the ring holds padded input rows y .. y+4 while output row y is reduced;
+    // registers xr[u] / dr[u] hold the rows that arrive DW_PF rows later
     for (int r = ya; r < ya + 4; r++) {
-        load_x(r);
-        store_x(r);
+        bf16x8 xr = {};
+        load_x(r, xr);
+        store_x(r, xr);
     }
-    load_x(ya + 4);
-    load_d(ya);
-    for (int y = ya; y < yb; y++) {
-        store_x(y + 4);
-        store_d();
-        __syncthreads();
-        if (y + 1 < yb) {
-            load_x(y + 5);
-            load_d(y + 1);
+    bf16x8 xr[DW_PF], dr[DW_PF];
+#pragma unroll
+    for (int u = 0; u < DW_PF; u++) {
+        xr[u] = bf16x8{};
+        dr[u] = bf16x8{};
+        if (ya + u < yb) {
+            load_x(ya + 4 + u, xr[u]);
+            load_d(ya + u, dr[u]);
         }
-        bf16x8 A[MB];
+    }
+    for (int y0 = ya; y0 < yb; y0 += DW_PF) {
 #pragma unroll
-        for (int mb = 0; mb < MB; mb++)
-            A[mb] = *reinterpret_cast<const bf16x8 *>(&dsh[mb * 16 + n16][kg]);
-#pragma unroll
-        for (int t = 0; t < MAXT; t++) {
-            const int tap = wv + 4 * t;
-            if (tap < TAPS) {  // wavefront-uniform
-                const int dy = tap / 5, dx = tap - 5 * (tap / 5);
-                const int slot = (y + dy) % 5;
-#pragma unroll
-                for (int nb = 0; nb < NB; nb++) {
-                    bf16x8 B;
-                    if (dx < 4) {
-                        B = *reinterpret_cast<const bf16x8 *>(&xs[slot][dx][nb * 16 + n16][kg]);
-                    } else {
-                        const bf16x4 lo =
-                            *reinterpret_cast<const bf16x4 *>(&xs[slot][0][nb * 16 + n16][4 + kg]);
-                        const bf16x4 hi =
-                            *reinterpret_cast<const bf16x4 *>(&xs[slot][0][nb * 16 + n16][8 + kg]);
-                        B = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
-                    }
-#pragma unroll
-                    for (int mb = 0; mb < MB; mb++)
-                        acc[t][mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                            A[mb], B, acc[t][mb][nb], 0, 0, 0);
+        for (int u = 0; u < DW_PF; u++) {
+            const int y = y0 + u;
+            if (y < yb) {  // workgroup-uniform
+                store_x(y + 4, xr[u]);
+                store_d(dr[u]);
+                __syncthreads();
+                if (y + DW_PF < yb) {
+                    load_x(y + 4 + DW_PF, xr[u]);
+                    load_d(y + DW_PF, dr[u]);
                 }
+                bf16x8 A[MB];
+#pragma unroll
+                for (int mb = 0; mb < MB; mb++)
+                    A[mb] = *reinterpret_cast<const bf16x8 *>(&dsh[mb * 16 + n16][kg]);
+#pragma unroll
+                for (int t = 0; t < MAXT; t++) {
+                    const int tap = wv + 4 * t;
+                    if (tap < TAPS) {  // wavefront-uniform
+                        const int dy = tap / 5, dx = tap - 5 * (tap / 5);
+                        const int slot = (y + dy) % 5;
+#pragma unroll
+                        for (int nb = 0; nb < NB; nb++) {
+                            bf16x8 B;
+                            if (dx < 4) {
+                                B = *reinterpret_cast<const bf16x8 *>(
+                                    &xs[slot][dx][nb * 16 + n16][kg]);
+                            } else {
+                                const bf16x4 lo = *reinterpret_cast<const bf16x4 *>(
+                                    &xs[slot][0][nb * 16 + n16][4 + kg]);
+                                const bf16x4 hi = *reinterpret_cast<const bf16x4 *>(
+                                    &xs[slot][0][nb * 16 + n16][8 + kg]);
+                                B = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+                            }
+#pragma unroll
+                            for (int mb = 0; mb < MB; mb++)
+                                acc[t][mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                    A[mb], B, acc[t][mb][nb], 0, 0, 0);
+                        }
+                    }
+                }
+                __syncthreads();
             }
         }
-        __syncthreads();
     }
     float *out = partial + (size_t)blockIdx.x * TAPS * CO * CI;
 #pragma unroll
@@ -642,14 +660,25 @@ __global__ __launch_bounds__(256) void k_bias_partial(const __bf16 *__restrict__
     }
 }
 
-__global__ void k_reduce(const float *__restrict__ partial, int nblk, int n,
-                         float *__restrict__ out)
+// out[i] = sum over the nblk partials: a workgroup owns 64 consecutive
+// outputs, its wavefront w sums the w-th quarter of the partials (64-lane
+// coalesced rows, 8 loads in flight), and the 4 quarter sums are added in
+// order, so the result does not depend on timing.
+__global__ __launch_bounds__(256) void k_reduce(const float *__restrict__ partial, int nblk,
+                                                int n, float *__restrict__ out)
 {
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i >= n) return;
+    __shared__ float red[4][64];
+    const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
+    const int i = (int)blockIdx.x * 64 + lane;
+    const int q = (nblk + 3) / 4, b0 = w * q, b1 = min(b0 + q, nblk);
     float s = 0.0f;
-    for (int b = 0; b < nblk; b++) s += partial[(size_t)b * n + i];
-    out[i] = s;
+    if (i < n) {
+#pragma unroll 8
+        for (int b = b0; b < b1; b++) s += partial[(size_t)b * n + i];
+    }
+    red[w][lane] = s;
+    __syncthreads();
+    if (w == 0 && i < n) out[i] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
 }
 
 // Global gradient norm, and the factor that scales it down to max_norm
@@ -989,12 +1018,13 @@ void launch_conv(hipStream_t s, int cin, int cout, Epilogue epi, const ConvArgs 
     NEMO_LAUNCH_CHECK();
 }
 
-// dW workgroups: nx / 32 column strips x row chunks, about 256 in all (at
-// least 8 rows a chunk, so the 4-row ring fill stays a small overhead)
+// dW workgroups: nx / 32 column strips x row chunks, about 512 in all, two
+// per CU (at least 8 rows a chunk, so the 4-row ring fill stays a small
+// overhead)
 static int dw_rows_per_block(int nx, int ny)
 {
     const long nstrip = nx / 32;
-    return (int)std::max(8L, (ny * nstrip + 255) / 256);
+    return (int)std::max(8L, (ny * nstrip + 511) / 512);
 }
 
 int dw_blocks(int nx, int ny)
@@ -1038,7 +1068,7 @@ void launch_bias_partial(hipStream_t s, int co, const __bf16 *dpre, float *parti
 
 void launch_reduce(hipStream_t s, const float *partial, int nblk, int n, float *out)
 {
-    k_reduce<<<grid_for(n), dim3(256), 0, s>>>(partial, nblk, n, out);
+    k_reduce<<<dim3((unsigned)((n + 63) / 64)), dim3(256), 0, s>>>(partial, nblk, n, out);
     NEMO_LAUNCH_CHECK();
 }
 
