@@ -17,10 +17,15 @@ norm clipping) on the pair (input C_k+1, target F_k+1 - C_k+1). The coarse
 model then continues from its own corrected state (as at inference, energy
 guard on), and is reset to F_k+1 every r steps, r growing from 1 to
 --rollout over the run, so that the network learns to pull drifted states
-back toward the fine solution. The loss and the correction leave out the
-ssh of cells the step clamps to the tide, and an activity gate
-1 - exp(-(a / --gate-eps)^2), a = |u| + |v| + |ssh| in input units, scales
-the output so that water at rest gets no correction. Normalisation scales
+back toward the fine solution. A rollout whose step loss exceeds
+--reset-spike times the seed's running mean is reset at once: a state that
+far off would only feed the network gradients of its own drift. The saved
+weights are a moving average of the weights over the second half of the
+run (--ema decay per sample, one sample every --ema-every steps). The
+loss and the correction leave out the ssh of cells the step clamps to the
+tide, and an activity gate 1 - exp(-(a / --gate-eps)^2), a = |u| + |v| +
+|ssh| in input units, scales the output so that water at rest gets no
+correction. Normalisation scales
 come from a pilot run of the first seed.
 
 Evaluation: for every held-out seed, the fine run, the plain coarse run and
@@ -157,6 +162,8 @@ def train(a) -> None:
     done = 0
     history = []
     spin_rng = np.random.default_rng(a.seed + 1)
+    ema = None                    # moving average of the weights (--ema)
+    spikes = 0
     for b in bathy:
         fine, coarse = pair(a.n, a.factor, b, a.steps)
         sf, sc = TimeStepper(fine, mode=2), TimeStepper(coarse, mode=2)
@@ -179,6 +186,7 @@ def train(a) -> None:
         net.set_state(sc)         # the window starts from the filtered fine state
         losses = []
         since_reset = 0
+        loss_avg = None
         for k in range(a.steps):
             frac = done / max(total - 1, 1)
             lr = a.lr_min + 0.5 * (a.lr - a.lr_min) * (1.0 + np.cos(np.pi * frac))
@@ -189,12 +197,26 @@ def train(a) -> None:
             net.train_step(sc)
             rollout = 1 + int(round((a.rollout - 1) * frac))
             since_reset += 1
-            if since_reset >= rollout:
+            spike = False
+            if a.reset_spike > 0:
+                # a rollout whose loss jumps far above this seed's running
+                # mean has drifted: restart it from the filtered fine state
+                # instead of training on where it went
+                loss = net.last_loss()
+                spike = loss_avg is not None and k >= 10 and loss > a.reset_spike * loss_avg
+                if not spike:
+                    loss_avg = loss if loss_avg is None else 0.95 * loss_avg + 0.05 * loss
+                else:
+                    spikes += 1
+            if since_reset >= rollout or spike:
                 net.set_state(sc)
                 since_reset = 0
             else:
                 net.correct(sc, guard=True)
             done += 1
+            if a.ema > 0 and frac >= a.ema_start and done % a.ema_every == 0:
+                p = net.params()
+                ema = p.astype(np.float64) if ema is None else a.ema * ema + (1.0 - a.ema) * p
             if (k + 1) % a.log_every == 0:
                 st = net.last_stats()
                 losses.append([st["loss_u"], st["loss_v"], st["loss_ssh"], st["grad_norm"]])
@@ -206,10 +228,12 @@ def train(a) -> None:
             f"{time.perf_counter() - t0:.0f} s")
         sf.release()
         sc.release()
-        params = net.params()
+        params = net.params() if ema is None else ema.astype(np.float32)
         save(a.out, params, norm, a, history)
     net.release()
-    log(f"trained {done} steps in {time.perf_counter() - t0:.0f} s -> {a.out}")
+    log(f"trained {done} steps in {time.perf_counter() - t0:.0f} s -> {a.out}"
+        + (f" (moving average of the weights, decay {a.ema})" if ema is not None else "")
+        + (f", {spikes} rollouts restarted on a loss spike" if a.reset_spike > 0 else ""))
 
 
 def save(path, params, norm, a, history) -> None:
@@ -217,6 +241,8 @@ def save(path, params, norm, a, history) -> None:
             "coarse_steps_per_seed": a.steps, "lr": a.lr, "lr_min": a.lr_min,
             "rollout": a.rollout, "guard_rel": a.guard_rel, "clip": a.clip,
             "max_spin": a.max_spin, "loss_weights": list(a.loss_weights),
+            "ema": a.ema, "ema_every": a.ema_every, "ema_start": a.ema_start,
+            "reset_spike": a.reset_spike,
             "optimizer": "adam", "history": history}
     Path(path).parent.mkdir(parents=True, exist_ok=True)
     np.savez(path, params=np.asarray(params, np.float32), meta=json.dumps(meta))
@@ -364,6 +390,16 @@ def main(argv=None) -> None:
     t.add_argument("--max-spin", type=int, default=1500,
                    help="longest spin-up of a seed's fine run before its window, in "
                         "coarse steps")
+    t.add_argument("--ema", type=float, default=0.99,
+                   help="save a moving average of the weights with this decay per sample "
+                        "(0: the last weights)")
+    t.add_argument("--ema-every", type=int, default=10,
+                   help="training steps between samples of the moving average")
+    t.add_argument("--ema-start", type=float, default=0.5,
+                   help="fraction of the run after which the average starts")
+    t.add_argument("--reset-spike", type=float, default=4.0,
+                   help="restart a rollout from the filtered fine state when a step's loss "
+                        "exceeds this multiple of the seed's running mean (0: never)")
     t.add_argument("--resume", default=None)
     t.add_argument("--out", default=str(DEFAULT_WEIGHTS))
     e = sub.add_parser("eval")

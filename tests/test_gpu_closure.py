@@ -234,3 +234,26 @@ def test_shipped_weights_beat_the_plain_coarse_model():
     assert s["guard_trips"] == 0
     for f in ("un", "vn", "sshn_t_free"):
         assert res["ratio_final"][f] < 0.8, f
+
+
+def test_training_command_end_to_end(tmp_path):
+    """The `closure_run train` command at a small size (fine 256^2, coarse
+    64^2): pilot normalisation, spin-ups, rollouts with spike restarts, the
+    moving average of the weights, the saved file, and an evaluation of it."""
+    import json as _json
+
+    from psyclonebench_amd import closure_run as cr
+    out = tmp_path / "w.npz"
+    print("closure train command: fine 256^2, 2 seeds x 30 steps", flush=True)
+    cr.main(["train", "--n", "256", "--bathy", "0:2", "--steps", "30", "--pilot-steps", "100",
+             "--max-spin", "5", "--rollout", "5", "--ema", "0.9", "--ema-every", "2",
+             "--ema-start", "0", "--reset-spike", "4", "--log-every", "10", "--out", str(out)])
+    z = np.load(out)
+    meta = _json.loads(str(z["meta"]))
+    assert z["params"].shape == (_native.hip_lib().nemo_cl_param_count(),)
+    assert np.all(np.isfinite(z["params"]))
+    assert meta["ema"] == 0.9 and meta["reset_spike"] == 4.0
+    assert len(meta["history"]) == 2
+    res = cr.evaluate(256, 4, [2], 20, 10, out, log_fn=lambda *x: None)
+    for f in ("sshn_t", "un", "vn"):
+        assert math.isfinite(res["ratio_mean"][f]), f
