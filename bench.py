@@ -35,7 +35,7 @@ METRIC = ("grid-pt updates/s + % HBM peak at 4096²; "
           "FP32/learned vs plain FP64, same err")
 HBM_PEAK_GBS = 8000.0
 B_ALG_F64 = 212          # bytes per interior grid-point update, FP64
-B_ALG_MIXED = 116        # FP32 fields + the ssh compensation word
+B_ALG_MIXED = 113        # FP32 fields, the byte mask and the ssh compensation word
 PROFILE_TRAFFIC = ROOT / "profiles" / "step_kernels_4096_traffic.md"
 
 
@@ -123,7 +123,7 @@ def rel_err(x, ref):
 
 def mixed_leg(pristine, a, pts, f64_ms_per_step, f64_kernel_ms, f64_variant) -> dict:
     """The FP32 mixed form on the same workload: throughput (same timing
-    protocol as the FP64 line), kernel roofline against its 116 B per
+    protocol as the FP64 line), kernel roofline against its 113 B per
     update, and its error against the FP64 form (bitwise the oracle) after
     250 / 500 / 1000 steps."""
     from psyclonebench_amd import _native

@@ -162,7 +162,8 @@ NEMO_API const char *nemo_hip_fused_variant_kernel(int variant);
  * the T-cell ssh is a compensated pair of FP32 words updated with FP64
  * accumulation; the tidal elevation amp*sin(omega_tide*istep*rdt) is FP64.
  * Takes the same 25 FP64 host arrays as c_invoke_time_step (converted on
- * upload) and returns an opaque handle. Requires the same mask condition as
+ * upload; the T mask is kept in one byte a point, so its values must lie in
+ * -64..63) and returns an opaque handle. Requires the same mask condition as
  * the fused form (exits otherwise). */
 NEMO_API void *nemo_mp_create(NEMO_HOST_ARRAYS, int internal_xstart,
                               int internal_xstop, int internal_ystart,
@@ -192,7 +193,8 @@ NEMO_API void nemo_mp_destroy(void *handle);
  *               width, total_size
  *   tide_amp, tide_omega   per domain: bc_ssh = amp*sin(omega*istep*rdt)
  *   precision   64: the FP64 step (bitwise the single-domain fused form);
- *               32: the mixed form (bitwise the single-domain nemo_mp_*)
+ *               32: the mixed form (bitwise the single-domain nemo_mp_*;
+ *               mask values in -64..63)
  * Every domain's mask must allow the fused form (no wet cell on row ye+1).
  * One field's arena (every domain, offsets rounded to 64 elements) must stay
  * below 4 GiB. Returns a handle; errors exit(1). */

@@ -43,6 +43,7 @@ struct EnsContext {
     long long interior = 0;  // interior T points over all domains
     EnsDomain *ddom = nullptr;
     int *tm = nullptr;
+    signed char *tm8 = nullptr;  // the mask in one byte a point (FP32 ensembles)
     // constant arenas by C-ABI index (A_HT, ..., A_DY_T) + Coriolis factors
     void *cst[A_COUNT] = {};
     void *fcor_u = nullptr, *fcor_v = nullptr;
@@ -122,7 +123,7 @@ struct EnsContext {
         f.dy_t = static_cast<const float *>(cst[A_DY_T]);
         f.fcor_u = static_cast<const float *>(fcor_u);
         f.fcor_v = static_cast<const float *>(fcor_v);
-        f.tmask = tm;
+        f.tmask = tm8;
         return f;
     }
     MixedOut out32(int s) const
@@ -142,7 +143,7 @@ struct EnsContext {
             for (void *p : set)
                 if (p) (void)hipFree(p);
         for (auto &kv : lists) (void)hipFree(kv.second.first);
-        void *more[] = {ddom, tm, fcor_u, fcor_v, d_tide};
+        void *more[] = {ddom, tm, tm8, fcor_u, fcor_v, d_tide};
         for (void *p : more)
             if (p) (void)hipFree(p);
         if (h_tide) (void)hipHostFree(h_tide);
@@ -449,6 +450,7 @@ void *nemo_ens_create(int ndom, void *const *arrays, const int *geom,
     C->fcor_u = arena(*C, es, s);
     C->fcor_v = arena(*C, es, s);
     C->tm = static_cast<int *>(arena(*C, sizeof(int), s));
+    if (precision == 32) C->tm8 = static_cast<signed char *>(arena(*C, 1, s));
     for (int set = 0; set < 2; set++)
         for (int k = 0; k < kProg; k++)
             if (k != 1 || precision == 32) C->prog[set][k] = arena(*C, es, s);
@@ -460,6 +462,13 @@ void *nemo_ens_create(int ndom, void *const *arrays, const int *geom,
         for (Arr a : kConsts) put(*C, C->cst[a], d, host(a), s, tmp);
         NEMO_HIP_CHECK(hipMemcpyAsync(C->tm + C->hdom[d].off, arrays[d * A_COUNT + A_TMASK],
                                       (size_t)G.total * sizeof(int), hipMemcpyHostToDevice, s));
+        std::vector<signed char> tm8;
+        if (C->tm8) {
+            tm8 = mask_bytes(static_cast<const int *>(arrays[d * A_COUNT + A_TMASK]),
+                             (size_t)G.total);
+            NEMO_HIP_CHECK(hipMemcpyAsync(C->tm8 + C->hdom[d].off, tm8.data(),
+                                          (size_t)G.total, hipMemcpyHostToDevice, s));
+        }
         std::vector<double> fu = coriolis(host(A_GPHIU), G.total, omega, d2r);
         std::vector<double> fv = coriolis(host(A_GPHIV), G.total, omega, d2r);
         put(*C, C->fcor_u, d, fu.data(), s, tmp);
@@ -476,7 +485,7 @@ void *nemo_ens_create(int ndom, void *const *arrays, const int *geom,
             put(*C, C->prog[set][4], d, host(A_SSHN_U), s, tmp);
             put(*C, C->prog[set][5], d, host(A_SSHN_V), s, tmp);
         }
-        NEMO_HIP_CHECK(hipStreamSynchronize(s));  // fu / fv go out of scope
+        NEMO_HIP_CHECK(hipStreamSynchronize(s));  // fu / fv / tm8 go out of scope
     }
     C->ddom = dev_alloc<EnsDomain>((size_t)ndom);
     NEMO_HIP_CHECK(hipMemcpyAsync(C->ddom, C->hdom.data(), ndom * sizeof(EnsDomain),

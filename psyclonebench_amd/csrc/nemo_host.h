@@ -144,6 +144,22 @@ inline std::vector<double> coriolis(const double *gphi, int n, double omega, dou
     return f;
 }
 
+// The T mask as one signed byte a point, for the FP32 kernels (a quarter of
+// the int32 mask's bytes). The step only compares mask values and adds two of
+// them, so a byte copy gives the same results for every value in -64..63; the
+// reference's masks hold -1, 0 and 1.
+inline std::vector<signed char> mask_bytes(const int *tm, size_t n)
+{
+    std::vector<signed char> b(n);
+    for (size_t k = 0; k < n; k++) {
+        if (tm[k] < -64 || tm[k] > 63)
+            NEMO_FATAL("T mask value %d at element %zu: the FP32 step keeps the mask "
+                       "in one byte (-64..63)", tm[k], k);
+        b[k] = (signed char)tm[k];
+    }
+    return b;
+}
+
 // The fused form copies nothing; it shares one buffer per after/now pair.
 // That reproduces every value the reference computes as long as no later
 // sweep reads a now-field outside the reference's copy region

@@ -29,7 +29,7 @@ struct MixedIn {
     const float *__restrict__ dy_t;
     const float *__restrict__ fcor_u;
     const float *__restrict__ fcor_v;
-    const int *__restrict__ tmask;
+    const signed char *__restrict__ tmask;  // one byte a point (mask_bytes)
 };
 
 struct MixedOut {

@@ -2,8 +2,9 @@
 // arithmetic for velocities, face ssh, depths and metrics; the T-cell ssh is
 // a compensated pair (hi, lo) of FP32 words whose update is accumulated in
 // FP64, and the tidal elevation is evaluated in FP64 on the host. Bytes per
-// interior update: 21 x 4 (fields, metrics) + 4 (tmask) + 5 x 4 (writes)
-// + 4 + 4 (the lo word read and written) = 116, against 212 in FP64.
+// interior update: 21 x 4 (fields, metrics) + 1 (the T mask, one byte a
+// point) + 5 x 4 (writes) + 4 + 4 (the lo word read and written) = 113,
+// against 212 in FP64.
 //
 // The arithmetic is nemo_point.h's *_val functions instantiated for float,
 // except continuity, whose flux divergence is formed in FP32 (the same

@@ -22,7 +22,7 @@ struct MixedContext {
     double amp = 0.0, omega_tide = 0.0;
     int rows = 32;
     float *cst[A_COUNT] = {};        // constant arrays (FP32); tmask in tm
-    int *tm = nullptr;
+    signed char *tm = nullptr;       // T mask, one byte a point
     float *fcor_u = nullptr, *fcor_v = nullptr;
     float *t_hi[2] = {}, *t_lo[2] = {}, *u[2] = {}, *v[2] = {}, *su[2] = {}, *sv[2] = {};
     DeviceFlather fl_u, fl_v;
@@ -164,6 +164,7 @@ void *nemo_mp_create(NEMO_HOST_ARRAYS, int internal_xstart, int internal_xstop,
     check_geometry(G);
     if (!fused_allowed(tmask, G))
         NEMO_FATAL("mixed form: the mask has wet cells on row %d", G.ye + 1);
+    const std::vector<signed char> tm8 = mask_bytes(tmask, (size_t)total_size);
     auto C = std::make_unique<MixedContext>();
     C->G = G;
     C->P = Physics{rdt, cbfr, visc, omega, d2r, g};
@@ -174,8 +175,9 @@ void *nemo_mp_create(NEMO_HOST_ARRAYS, int internal_xstart, int internal_xstop,
     const Arr consts[] = {A_HT, A_HU, A_HV, A_SSHA_U, A_SSHA_V, A_AREA_T, A_AREA_U,
                           A_AREA_V, A_DX_U, A_DX_V, A_DX_T, A_DY_U, A_DY_V, A_DY_T};
     for (Arr a : consts) C->cst[a] = upload_f32(host[a], n, s);
-    C->tm = dev_alloc<int>(n);
-    NEMO_HIP_CHECK(hipMemcpyAsync(C->tm, tmask, n * sizeof(int), hipMemcpyHostToDevice, s));
+    C->tm = dev_alloc<signed char>(n);
+    NEMO_HIP_CHECK(hipMemcpyAsync(C->tm, tm8.data(), n, hipMemcpyHostToDevice, s));
+    NEMO_HIP_CHECK(hipStreamSynchronize(s));  // the copy reads tm8 (pageable host memory)
     std::vector<double> fu = coriolis(gphiu, total_size, omega, d2r);
     std::vector<double> fv = coriolis(gphiv, total_size, omega, d2r);
     C->fcor_u = upload_f32(fu.data(), n, s);

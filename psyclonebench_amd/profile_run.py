@@ -71,7 +71,7 @@ def main() -> None:
             st.run(1)
             out["mixed_kernel_ms"][rows] = st.timed_run(a.mixed)
             st.release()
-        out["mixed_alg_bytes_per_launch"] = 116 * (cfg.jpiglo - 2) * (cfg.jpjglo - 2)
+        out["mixed_alg_bytes_per_launch"] = 113 * (cfg.jpiglo - 2) * (cfg.jpjglo - 2)
     print(json.dumps(out), flush=True)
 
 

@@ -144,3 +144,23 @@ def test_setup_library_exports_every_declared_symbol():
 def test_device_info_without_gpu_does_not_exit():
     n, text = _native.device_info()
     assert n >= 0 and isinstance(text, str)
+
+
+def test_mixed_form_rejects_mask_values_beyond_a_byte():
+    """The FP32 step keeps the T mask in one byte: nemo_mp_create rejects a
+    value outside -64..63 on the host, before any device call, with the
+    library's error convention (message, exit(1))."""
+    import subprocess
+    import sys
+    code = (
+        "from psyclonebench_amd.model import Model, c1_config\n"
+        "from psyclonebench_amd.psy import MixedStepper\n"
+        "m = Model(c1_config(10))\n"
+        "m.array('tmask')[5, 5] = 100\n"
+        "MixedStepper(m)\n"
+    )
+    root = Path(__file__).resolve().parents[1]
+    r = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 1, (r.returncode, r.stderr[-2000:])
+    assert "T mask value 100" in r.stderr + r.stdout, r.stderr[-2000:]
