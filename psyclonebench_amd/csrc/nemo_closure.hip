@@ -683,15 +683,19 @@ __global__ __launch_bounds__(256) void k_reduce(const float *__restrict__ partia
 
 // Global gradient norm, and the factor that scales it down to max_norm
 // (1 when max_norm <= 0 or the norm is below it): norm_scale = {norm, factor}.
-__global__ __launch_bounds__(256) void k_grad_norm(const float *__restrict__ g, int n,
-                                                   float max_norm, float *__restrict__ norm_scale)
+// One workgroup of 1024 threads, 8 loads in flight per thread, then a tree
+// in LDS: a fixed order, and a few memory latencies for the ~77K values.
+__global__ __launch_bounds__(1024) void k_grad_norm(const float *__restrict__ g, int n,
+                                                    float max_norm,
+                                                    float *__restrict__ norm_scale)
 {
-    __shared__ float red[256];
+    __shared__ float red[1024];
     float s = 0.0f;
-    for (int i = (int)threadIdx.x; i < n; i += 256) s += g[i] * g[i];
+#pragma unroll 8
+    for (int i = (int)threadIdx.x; i < n; i += 1024) s += g[i] * g[i];
     red[threadIdx.x] = s;
     __syncthreads();
-    for (int k = 128; k > 0; k >>= 1) {
+    for (int k = 512; k > 0; k >>= 1) {
         if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
         __syncthreads();
     }
@@ -1076,7 +1080,7 @@ void launch_optimizer(hipStream_t s, int kind, float *w, const float *g, float *
                       float *v, int n, float lr, float b1, float b2, float eps,
                       float bc1, float bc2, float max_norm, float *norm_scale)
 {
-    k_grad_norm<<<dim3(1), dim3(256), 0, s>>>(g, n, max_norm, norm_scale);
+    k_grad_norm<<<dim3(1), dim3(1024), 0, s>>>(g, n, max_norm, norm_scale);
     NEMO_LAUNCH_CHECK();
     k_optimizer<<<grid_for(n), dim3(256), 0, s>>>(kind, w, g, m, v, n, lr, b1, b2, eps,
                                                   bc1, bc2, norm_scale);

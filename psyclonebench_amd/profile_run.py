@@ -7,7 +7,10 @@ Runs `--fused` fused steps of the 4096^2 FP64 workload as plain launches
 the unfused sequence, whose k_copy3 dispatch is a plain copy of known size
 (3 arrays read + 3 written over the copy region) used to calibrate the
 FETCH_SIZE / WRITE_SIZE counters. Prints the dispatch geometry and the
-algorithmic byte counts the profiles are compared with.
+algorithmic byte counts the profiles are compared with. `--closure-train`
+runs training steps of the learned closure (forward, loss, backward, Adam)
+on synthetic input / target pairs over the 1024^2 coarse grid of the same
+workload, uploaded from the host each step.
 """
 from __future__ import annotations
 
@@ -32,6 +35,8 @@ def main() -> None:
                    help="steps of the FP32 mixed form (plain launches)")
     p.add_argument("--mixed-rows", type=str, default="64",
                    help="rows per band of the mixed kernel, comma separated")
+    p.add_argument("--closure-train", type=int, default=0,
+                   help="closure training steps on synthetic coarse-grid pairs")
     a = p.parse_args()
     lib = _native.hip_lib()
     variants = [int(v) for v in a.variant.split(",")]
@@ -72,6 +77,22 @@ def main() -> None:
             out["mixed_kernel_ms"][rows] = st.timed_run(a.mixed)
             st.release()
         out["mixed_alg_bytes_per_launch"] = 113 * (cfg.jpiglo - 2) * (cfg.jpjglo - 2)
+    if a.closure_train:
+        import numpy as np
+
+        from .closure import ClosureNet, coarse_model
+        coarse = coarse_model(Model(cfg), 4)
+        g = coarse.grid
+        ny, nx = g.ye - g.ys + 1, g.xe - g.xs + 1
+        net = ClosureNet(coarse, seed=0)
+        net.set_optimizer("adam", lr=1e-4)
+        rng = np.random.default_rng(0)
+        x = rng.standard_normal((ny, nx, 4))
+        t = 0.1 * rng.standard_normal((ny, nx, 3))
+        losses = [net.train_host(x, t) for _ in range(a.closure_train)]
+        net.release()
+        out["closure_grid"] = [nx, ny]
+        out["closure_train_loss_first_last"] = [losses[0], losses[-1]]
     print(json.dumps(out), flush=True)
 
 
