@@ -332,6 +332,16 @@ void publish(const Context &C, void **slot[A_COUNT])
     }
 }
 
+// The read / write callbacks copy nx*ny doubles between two buffers; a
+// negative extent or a null buffer is a caller error, reported like the
+// others (message, exit(1)).
+void check_copy(const char *who, const void *from, const void *to, int nx, int ny)
+{
+    if (nx < 0 || ny < 0) NEMO_FATAL("%s: negative extent %d x %d", who, nx, ny);
+    if ((long)nx * ny > 0 && (!from || !to))
+        NEMO_FATAL("%s: null buffer (from %p, to %p)", who, from, to);
+}
+
 }  // namespace
 }  // namespace nemo
 
@@ -413,6 +423,7 @@ void c_read_from_device(void *from, void *to, int startx, int starty, int nx,
     (void)startx;
     (void)starty;
     (void)blocking;
+    check_copy("c_read_from_device", from, to, nx, ny);
     NEMO_HIP_CHECK(hipStreamSynchronize(lib_stream()));
     NEMO_HIP_CHECK(hipMemcpy(to, from, (size_t)nx * ny * sizeof(double),
                              hipMemcpyDeviceToHost));
@@ -424,6 +435,7 @@ void c_write_to_device(void *from, void *to, int startx, int starty, int nx,
     (void)startx;
     (void)starty;
     (void)blocking;
+    check_copy("c_write_to_device", from, to, nx, ny);
     NEMO_HIP_CHECK(hipStreamSynchronize(lib_stream()));
     NEMO_HIP_CHECK(hipMemcpy(to, from, (size_t)nx * ny * sizeof(double),
                              hipMemcpyHostToDevice));

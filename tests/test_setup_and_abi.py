@@ -164,3 +164,19 @@ def test_mixed_form_rejects_mask_values_beyond_a_byte():
                        text=True, timeout=120)
     assert r.returncode == 1, (r.returncode, r.stderr[-2000:])
     assert "T mask value 100" in r.stderr + r.stdout, r.stderr[-2000:]
+
+
+@pytest.mark.parametrize("call", ["c_read_from_device(None, None, 1, 1, -3, 4, True)",
+                                  "c_write_to_device(None, None, 1, 1, 8, 8, True)"])
+def test_copy_callbacks_reject_bad_arguments(call):
+    """A negative extent or a null buffer ends the process with the library's
+    message and exit(1), before any device call."""
+    import subprocess
+    import sys
+    code = ("from psyclonebench_amd import _native\n"
+            f"_native.hip_lib().{call}\n")
+    root = Path(__file__).resolve().parents[1]
+    r = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 1, (r.returncode, r.stderr[-2000:])
+    assert "negative extent" in r.stderr or "null buffer" in r.stderr, r.stderr[-2000:]
