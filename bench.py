@@ -133,11 +133,14 @@ def mixed_leg(pristine, a, pts, f64_ms_per_step, f64_kernel_ms, f64_variant) -> 
     st = MixedStepper(pristine.copy())
     st.run(1 + a.warmup)
     tune = {}
-    for rows in (16, 32, 48, 64, 80, 96, 128):
-        lib.nemo_mp_set_rows(st.handle, rows)
-        tune[rows] = st.timed_run(a.tune_steps)
-    rows = min(tune, key=tune.get)
+    for cols in (61, 56, 48):
+        lib.nemo_mp_set_columns(st.handle, cols)
+        for rows in (4, 8, 16, 32, 48, 64, 80, 96, 128):
+            lib.nemo_mp_set_rows(st.handle, rows)
+            tune[(rows, cols)] = st.timed_run(a.tune_steps)
+    rows, cols = min(tune, key=tune.get)
     lib.nemo_mp_set_rows(st.handle, rows)
+    lib.nemo_mp_set_columns(st.handle, cols)
     st.run(a.steps)
     lib.nemo_hip_synchronize()
     t0 = time.perf_counter()
@@ -152,7 +155,7 @@ def mixed_leg(pristine, a, pts, f64_ms_per_step, f64_kernel_ms, f64_variant) -> 
     sl = (slice(g.ys, g.ye + 1), slice(g.xs, g.xe + 1))
     lib.nemo_hip_set_fused_variant(f64_variant)
     ref = TimeStepper(pristine.copy(), mode=2)
-    mp = MixedStepper(pristine.copy(), rows=rows)
+    mp = MixedStepper(pristine.copy(), rows=rows, cols=cols)
     errs, done = {}, 0
     for cp in (250, 500, 1000):
         ref.run(cp - done)
@@ -173,11 +176,15 @@ def mixed_leg(pristine, a, pts, f64_ms_per_step, f64_kernel_ms, f64_variant) -> 
         "speedup_vs_f64": f64_ms_per_step / ms_step,
         "kernel_speedup_vs_f64": f64_kernel_ms / kernel_ms,
         "rows_per_band": rows,
-        "rows_kernel_ms": {str(k): round(v, 4) for k, v in tune.items()},
+        "columns_per_wavefront": cols,
+        "rows_kernel_ms": {f"{r}x{c}": round(v, 4) for (r, c), v in tune.items()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic_from_profile(f"mixed rows {rows}"),
-                     "kernel": "k_mixed_wave", "kernel_ms": kernel_ms,
+                     "traffic": traffic_from_profile(
+                         f"mixed rows {rows}" + ("" if cols == 61 else f" cols {cols}")),
+                     "kernel": (f"k_mixed_wave<{rows}>" if cols == 61
+                                else f"k_mixed_wave_a<{rows}, {cols}>"),
+                     "kernel_ms": kernel_ms,
                      "bytes_per_launch_alg": b_alg},
         "error_vs_f64": {"metric": "(relmax, relrms) over the internal T region",
                          "after_steps": errs},
@@ -367,8 +374,10 @@ def main():
             "hbm_frac_whole_step": B_ALG_F64 * updates_per_s / (HBM_PEAK_GBS * 1e9),
             "fused_variant": best,
             "fused_shape": {"block_columns": bw.value, "rows": rows.value,
-                            "kind": ["row march", "row march, loads one row ahead",
-                                     "overlapped-wavefront march"][min(pf.value, 2)]},
+                            "kind": (f"overlapped-wavefront march, {56 if pf.value == 5 else 48} "
+                                     "columns, 64-byte-aligned stores" if pf.value >= 5 else
+                                     ["row march", "row march, loads one row ahead",
+                                      "overlapped-wavefront march"][min(pf.value, 2)])},
             "variant_kernel_ms": {str(k): round(v, 4) for k, v in tune.items()},
         },
         "roofline": {

@@ -151,7 +151,9 @@ NEMO_API int nemo_hip_fused_variant(void);
 /* Block width (columns), rows per block, and the march kind of a variant:
  * 0 = row march, 1 = row march with the row loads issued one row ahead,
  * 2/3/4 = overlapped-wavefront march with at least 1/3/4 waves per SIMD
- * requested from the register allocator. */
+ * requested from the register allocator, 5 / 6 = overlapped-wavefront
+ * march with 56 / 48 output columns per wavefront (64-byte-aligned
+ * stores). */
 NEMO_API void nemo_hip_fused_variant_shape(int variant, int *block_width,
                                            int *rows, int *prefetch);
 /* Name of the kernel a variant launches, as profilers report it. */
@@ -176,8 +178,13 @@ NEMO_API void nemo_mp_steps(void *handle, int nsteps, int istep);
 /* The same steps as plain launches, each timed with HIP events; returns the
  * mean duration in ms of the mixed step kernel. */
 NEMO_API double nemo_mp_kernel_ms(void *handle, int nsteps, int istep);
-/* Rows per band of the kernel: 16, 32 (default), 48, 64, 80, 96 or 128. */
+/* Rows per band of the kernel: 4, 8, 16, 32 (default), 48, 64, 80, 96 or
+ * 128. */
 NEMO_API void nemo_mp_set_rows(void *handle, int rows);
+/* Output columns per wavefront: 61 (default; strips overlap by 3 columns),
+ * 56 (every strip's stores start on a 32-byte boundary) or 48 (on a 64-byte
+ * boundary). */
+NEMO_API void nemo_mp_set_columns(void *handle, int cols);
 /* Blocking read-back as FP64 (total_size values): field 0 sshn_t (hi+lo),
  * 1 un, 2 vn, 3 sshn_u, 4 sshn_v. */
 NEMO_API void nemo_mp_read(void *handle, int field, double *out);

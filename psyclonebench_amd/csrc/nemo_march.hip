@@ -447,6 +447,33 @@ __global__ __launch_bounds__(64, MINW) void k_fused_wave(
     wave::tile_f64<R>(f, p, o, W, 0u, b, strip, band, tide_tab ? tide_tab[tide_k] : tide_val);
 }
 
+// The same march with OW owned columns per wavefront, OW a multiple of 8, so
+// that every strip's stores start on a 64-byte boundary (OW = 56: the loads
+// start 32 bytes before one; OW = 48: on one).
+template <int R, int OW>
+__global__ __launch_bounds__(64) void k_fused_wave_a(
+    StepIn<double> f, StepScalars<double> p, StepOut<double> o, int W, Bounds b,
+    int nstrip, const double *__restrict__ tide_tab, int tide_k, double tide_val)
+{
+    static_assert(OW % 8 == 0, "64-byte-aligned strips");
+    const int logical = wave::xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const int band = logical / nstrip, strip = logical - band * nstrip;
+    wave::tile_f64<R, OW>(f, p, o, W, 0u, b, strip, band,
+                          tide_tab ? tide_tab[tide_k] : tide_val);
+}
+
+template <int R, int OW>
+void launch_wave_aligned_t(hipStream_t s, const StepIn<double> &f,
+                           const StepScalars<double> &p, const StepOut<double> &o, int W,
+                           Bounds b, const double *tide_tab, int tide_k, double tide_val)
+{
+    const int ncol = b.xe - b.xs + 3, nrow = b.ye - b.ys + 3;
+    const int nstrip = (ncol + OW - 1) / OW, nband = (nrow + R - 1) / R;
+    k_fused_wave_a<R, OW><<<dim3((unsigned)(nstrip * nband)), dim3(64), 0, s>>>(
+        f, p, o, W, b, nstrip, tide_tab, tide_k, tide_val);
+    NEMO_LAUNCH_CHECK();
+}
+
 template <int R, int MINW>
 void launch_wave_t(hipStream_t s, const StepIn<double> &f,
                    const StepScalars<double> &p, const StepOut<double> &o, int W,
@@ -462,7 +489,9 @@ void launch_wave_t(hipStream_t s, const StepIn<double> &f,
 // kind: 0 register-window march, 1 the same with the row loads issued one
 // row ahead, 2 overlapped-wavefront march (61 output columns per block),
 // 3 / 4 the overlapped-wavefront march compiled for >= 3 / 4 wavefronts per
-// SIMD (the compiler keeps its registers within that budget)
+// SIMD (the compiler keeps its registers within that budget), 5 / 6 the
+// overlapped-wavefront march with 56 / 48 output columns (64-byte-aligned
+// stores; with 48 the loads are aligned too)
 struct MarchShape {
     int mx, rows, prefetch;
 };
@@ -488,7 +517,9 @@ static const MarchShape kShapes[] = {
     {64, 64, 0},  {64, 128, 0}, {256, 32, 1}, {256, 64, 1}, {256, 16, 1},
     {128, 32, 1}, {64, 64, 1},  {64, 32, 1},  {64, 16, 2},  {64, 32, 2},
     {64, 64, 2},  {64, 128, 2}, {64, 32, 3},  {64, 64, 3},  {64, 32, 4},
-    {64, 64, 4},
+    {64, 64, 4},  {64, 16, 5},  {64, 32, 5},  {64, 8, 5},   {64, 16, 6},
+    {64, 32, 6},  {64, 4, 5},   {64, 8, 6},   {64, 4, 6},   {64, 2, 5},
+    {64, 6, 5},
 };
 
 // Kernel names as the profiler reports them (template arguments included).
@@ -503,7 +534,12 @@ static const char *const kNames[] = {
     "k_fused_wave<32, 1>",           "k_fused_wave<64, 1>",
     "k_fused_wave<128, 1>",          "k_fused_wave<32, 3>",
     "k_fused_wave<64, 3>",           "k_fused_wave<32, 4>",
-    "k_fused_wave<64, 4>",
+    "k_fused_wave<64, 4>",           "k_fused_wave_a<16, 56>",
+    "k_fused_wave_a<32, 56>",        "k_fused_wave_a<8, 56>",
+    "k_fused_wave_a<16, 48>",        "k_fused_wave_a<32, 48>",
+    "k_fused_wave_a<4, 56>",         "k_fused_wave_a<8, 48>",
+    "k_fused_wave_a<4, 48>",         "k_fused_wave_a<2, 56>",
+    "k_fused_wave_a<6, 56>",
 };
 static_assert(sizeof(kNames) / sizeof(kNames[0]) == sizeof(kShapes) / sizeof(kShapes[0]),
               "one kernel name per shape");
@@ -557,6 +593,36 @@ void launch_fused_march(hipStream_t s, int variant, const StepIn<double> &f,
     case 18: launch_wave_t<64, 3>(s, f, p, o, W, b, tide_tab, tide_k, tide_val); break;
     case 19: launch_wave_t<32, 4>(s, f, p, o, W, b, tide_tab, tide_k, tide_val); break;
     case 20: launch_wave_t<64, 4>(s, f, p, o, W, b, tide_tab, tide_k, tide_val); break;
+    case 21:
+        launch_wave_aligned_t<16, 56>(s, f, p, o, W, b, tide_tab, tide_k, tide_val);
+        break;
+    case 22:
+        launch_wave_aligned_t<32, 56>(s, f, p, o, W, b, tide_tab, tide_k, tide_val);
+        break;
+    case 23:
+        launch_wave_aligned_t<8, 56>(s, f, p, o, W, b, tide_tab, tide_k, tide_val);
+        break;
+    case 24:
+        launch_wave_aligned_t<16, 48>(s, f, p, o, W, b, tide_tab, tide_k, tide_val);
+        break;
+    case 25:
+        launch_wave_aligned_t<32, 48>(s, f, p, o, W, b, tide_tab, tide_k, tide_val);
+        break;
+    case 26:
+        launch_wave_aligned_t<4, 56>(s, f, p, o, W, b, tide_tab, tide_k, tide_val);
+        break;
+    case 27:
+        launch_wave_aligned_t<8, 48>(s, f, p, o, W, b, tide_tab, tide_k, tide_val);
+        break;
+    case 28:
+        launch_wave_aligned_t<4, 48>(s, f, p, o, W, b, tide_tab, tide_k, tide_val);
+        break;
+    case 29:
+        launch_wave_aligned_t<2, 56>(s, f, p, o, W, b, tide_tab, tide_k, tide_val);
+        break;
+    case 30:
+        launch_wave_aligned_t<6, 56>(s, f, p, o, W, b, tide_tab, tide_k, tide_val);
+        break;
     default: NEMO_FATAL("unknown fused march variant %d", variant);
     }
 #undef NEMO_MARCH_CASE

@@ -36,8 +36,10 @@ struct MixedOut {
     float *ssh_hi, *ssh_lo, *u, *v, *sshu, *sshv;
 };
 
-// rows per band: 16, 32, 64 or 128
-void launch_mixed_step(hipStream_t s, int rows, const MixedIn &f,
+// rows per band: 4, 8, 16, 32, 48, 64, 80, 96 or 128; output columns per
+// wavefront: 61 (overlapped strips), 56 (32-byte-aligned stores) or 48
+// (64-byte-aligned stores)
+void launch_mixed_step(hipStream_t s, int rows, int cols, const MixedIn &f,
                        const StepScalars<float> &p, const MixedOut &o, int W,
                        Bounds b, const double *tide_tab, int tide_k, double tide_val);
 void launch_flather_f32(hipStream_t s, const FlatherList &lu, const FlatherList &lv,

@@ -31,6 +31,22 @@ __global__ __launch_bounds__(64) void k_mixed_wave(MixedIn f, StepScalars<float>
                         wave::split(tide_tab ? tide_tab[tide_k] : tide_val));
 }
 
+// The same with OW owned columns per wavefront: 48 (192-byte strips, every
+// strip's stores start on a 64-byte boundary) or 56 (224-byte strips,
+// 32-byte boundaries, fewer neighbour lanes).
+template <int R, int OW>
+__global__ __launch_bounds__(64) void k_mixed_wave_a(MixedIn f, StepScalars<float> p,
+                                                     MixedOut o, int W, Bounds b,
+                                                     int nstrip,
+                                                     const double *__restrict__ tide_tab,
+                                                     int tide_k, double tide_val)
+{
+    const int logical = wave::xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const int band = logical / nstrip, strip = logical - band * nstrip;
+    wave::tile_mixed<R, OW>(f, p, o, W, 0u, b, strip, band,
+                            wave::split(tide_tab ? tide_tab[tide_k] : tide_val));
+}
+
 // Flather over the same segment lists as the FP64 path, FP32 values.
 __global__ __launch_bounds__(64) void k_flather_f32(FlatherList lu, FlatherList lv,
                                                     float *__restrict__ ua,
@@ -88,20 +104,31 @@ inline dim3 blocks_for(long n) { return dim3((unsigned)((n + 255) / 256)); }
 
 }  // namespace
 
-void launch_mixed_step(hipStream_t s, int rows, const MixedIn &f,
+void launch_mixed_step(hipStream_t s, int rows, int cols, const MixedIn &f,
                        const StepScalars<float> &p, const MixedOut &o, int W,
                        Bounds b, const double *tide_tab, int tide_k, double tide_val)
 {
+    if (cols != wave::OUT && cols != 56 && cols != 48)
+        NEMO_FATAL("mixed step: unsupported columns per wavefront %d", cols);
     const int ncol = b.xe - b.xs + 3, nrow = b.ye - b.ys + 3;
-    const int nstrip = (ncol + wave::OUT - 1) / wave::OUT;
+    const int nstrip = (ncol + cols - 1) / cols;
     switch (rows) {
-#define NEMO_MIXED_CASE(RR)                                                       \
-    case RR: {                                                                    \
-        const int nband = (nrow + RR - 1) / RR;                                   \
-        k_mixed_wave<RR><<<dim3((unsigned)(nstrip * nband)), dim3(64), 0, s>>>( \
-            f, p, o, W, b, nstrip, tide_tab, tide_k, tide_val);                   \
-        break;                                                                    \
+#define NEMO_MIXED_CASE(RR)                                                          \
+    case RR: {                                                                       \
+        const dim3 grid((unsigned)(nstrip * ((nrow + RR - 1) / RR)));                \
+        if (cols == 48)                                                              \
+            k_mixed_wave_a<RR, 48><<<grid, dim3(64), 0, s>>>(                        \
+                f, p, o, W, b, nstrip, tide_tab, tide_k, tide_val);                  \
+        else if (cols == 56)                                                         \
+            k_mixed_wave_a<RR, 56><<<grid, dim3(64), 0, s>>>(                        \
+                f, p, o, W, b, nstrip, tide_tab, tide_k, tide_val);                  \
+        else                                                                         \
+            k_mixed_wave<RR><<<grid, dim3(64), 0, s>>>(f, p, o, W, b, nstrip,        \
+                                                       tide_tab, tide_k, tide_val);   \
+        break;                                                                       \
     }
+        NEMO_MIXED_CASE(4)
+        NEMO_MIXED_CASE(8)
         NEMO_MIXED_CASE(16)
         NEMO_MIXED_CASE(32)
         NEMO_MIXED_CASE(48)

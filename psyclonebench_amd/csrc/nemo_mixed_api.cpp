@@ -21,6 +21,7 @@ struct MixedContext {
     Physics P{};
     double amp = 0.0, omega_tide = 0.0;
     int rows = 32;
+    int cols = 61;                   // output columns per wavefront
     float *cst[A_COUNT] = {};        // constant arrays (FP32); tmask in tm
     signed char *tm = nullptr;       // T mask, one byte a point
     float *fcor_u = nullptr, *fcor_v = nullptr;
@@ -30,7 +31,7 @@ struct MixedContext {
     bool paired = false;
     double *d_tide = nullptr, *h_tide = nullptr;
     int tide_cap = 0;
-    std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;  // n, parity, rows
+    std::map<std::tuple<int, int, int, int>, hipGraphExec_t> graphs;  // n, parity, rows, cols
     std::vector<hipEvent_t> ev;
 
     MixedIn inputs(int s) const
@@ -106,7 +107,7 @@ void launch_pair(MixedContext &C, int par, hipStream_t s, const double *tide_tab
     const int q = 1 - par;
     const MixedOut o{C.t_hi[q], C.t_lo[q], C.u[q], C.v[q], C.su[q], C.sv[q]};
     if (e0) NEMO_HIP_CHECK(hipEventRecord(e0, s));
-    launch_mixed_step(s, C.rows, C.inputs(par), C.scalars(), o, C.G.W,
+    launch_mixed_step(s, C.rows, C.cols, C.inputs(par), C.scalars(), o, C.G.W,
                       Bounds{C.G.xs, C.G.xe, C.G.ys, C.G.ye}, tide_tab, k, tide_val);
     if (e1) NEMO_HIP_CHECK(hipEventRecord(e1, s));
     launch_flather_f32(s, C.fl_u.view(), C.fl_v.view(), C.u[q], C.v[q], C.su[par],
@@ -210,11 +211,19 @@ void *nemo_mp_create(NEMO_HOST_ARRAYS, int internal_xstart, int internal_xstop,
 
 void nemo_mp_set_rows(void *h, int rows)
 {
-    if (rows != 16 && rows != 32 && rows != 48 && rows != 64 && rows != 80 && rows != 96 &&
-        rows != 128)
-        NEMO_FATAL("mixed form: rows per band must be 16, 32, 48, 64, 80, 96 or 128 (got %d)",
-                   rows);
+    if (rows != 4 && rows != 8 && rows != 16 && rows != 32 && rows != 48 && rows != 64 &&
+        rows != 80 && rows != 96 && rows != 128)
+        NEMO_FATAL("mixed form: rows per band must be 4, 8, 16, 32, 48, 64, 80, 96 or 128 "
+                   "(got %d)", rows);
     static_cast<MixedContext *>(h)->rows = rows;
+}
+
+void nemo_mp_set_columns(void *h, int cols)
+{
+    if (cols != 61 && cols != 56 && cols != 48)
+        NEMO_FATAL("mixed form: output columns per wavefront must be 61, 56 or 48 (got %d)",
+                   cols);
+    static_cast<MixedContext *>(h)->cols = cols;
 }
 
 static void mp_run(MixedContext &C, int nsteps, int istep, bool timed, double *ms)
@@ -245,7 +254,7 @@ static void mp_run(MixedContext &C, int nsteps, int istep, bool timed, double *m
         }
         *ms = sum / nsteps;
     } else {
-        auto key = std::make_tuple(nsteps, C.parity, C.rows);
+        auto key = std::make_tuple(nsteps, C.parity, C.rows, C.cols);
         auto it = C.graphs.find(key);
         if (it == C.graphs.end()) {
             hipGraph_t graph;

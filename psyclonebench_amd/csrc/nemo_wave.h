@@ -34,12 +34,41 @@ __device__ __forceinline__ bool in_rect(int i, int j, int x0, int x1, int y0,
     return i >= x0 && i <= x1 && j >= y0 && j <= y1;
 }
 
-__device__ __forceinline__ double from_below(double x) { return __shfl_up(x, 1, 64); }
-__device__ __forceinline__ double from_above(double x) { return __shfl_down(x, 1, 64); }
-__device__ __forceinline__ float from_below(float x) { return __shfl_up(x, 1, 64); }
-__device__ __forceinline__ float from_above(float x) { return __shfl_down(x, 1, 64); }
-__device__ __forceinline__ int from_below(int x) { return __shfl_up(x, 1, 64); }
-__device__ __forceinline__ int from_above(int x) { return __shfl_down(x, 1, 64); }
+// Neighbour exchange across the wavefront by DPP wave shifts, a modifier of
+// a VALU move (no LDS traffic, no wait): from_below gives lane i the value
+// of lane i-1 (wave_shr:1), from_above that of lane i+1 (wave_shl:1). Lane 0
+// (resp. 63) receives 0; those lanes never own a column, and no owned value
+// depends on what they receive. Called in wavefront-uniform control flow.
+__device__ __forceinline__ int from_below(int x)
+{
+    return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, true);
+}
+__device__ __forceinline__ int from_above(int x)
+{
+    return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xf, 0xf, true);
+}
+__device__ __forceinline__ float from_below(float x)
+{
+    return __builtin_bit_cast(float, from_below(__builtin_bit_cast(int, x)));
+}
+__device__ __forceinline__ float from_above(float x)
+{
+    return __builtin_bit_cast(float, from_above(__builtin_bit_cast(int, x)));
+}
+__device__ __forceinline__ double from_below(double x)
+{
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+    const unsigned lo = (unsigned)from_below((int)(unsigned)u);
+    const unsigned hi = (unsigned)from_below((int)(unsigned)(u >> 32));
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ double from_above(double x)
+{
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+    const unsigned lo = (unsigned)from_above((int)(unsigned)u);
+    const unsigned hi = (unsigned)from_above((int)(unsigned)(u >> 32));
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
 
 // Strips and bands of the extent the fused step walks: columns xs-1 .. xe+1,
 // rows ys-1 .. ye+1.
@@ -57,8 +86,13 @@ __device__ __forceinline__ int xcd_tile(int bid, int nblk)
 }
 
 // One FP64 tile: new ssh (continuity + bc_ssh), momentum + bc_solid for U
-// and V, next_sshu/v, stored where the lane owns its column.
-template <int R>
+// and V, next_sshu/v, stored where the lane owns its column. The wavefront
+// owns OW columns, lanes L0 .. L0+OW-1 (L0 = (64 - OW) / 2); the lanes on
+// either side load the neighbour columns (the next_ssh of the last owned
+// column needs the new ssh of the column after it, hence two lanes above).
+// OW = 56 with L0 = 4 starts every strip's stores on a 64-byte boundary
+// (56 doubles = 448 bytes), at the price of 8 shared columns instead of 3.
+template <int R, int OW = OUT>
 __device__ __forceinline__ void tile_f64(const StepIn<double> &f,
                                          const StepScalars<double> &p,
                                          const StepOut<double> &o, int W,
@@ -66,11 +100,13 @@ __device__ __forceinline__ void tile_f64(const StepIn<double> &f,
                                          const Bounds &b, int strip, int band,
                                          double tide)
 {
+    constexpr int L0 = (64 - OW) / 2;
+    static_assert(L0 >= 1 && L0 + OW + 2 <= 64, "one neighbour lane below, two above");
     using T = double;
     const int xmax = b.xe + 1, ymax = b.ye + 1;
     const int lane = (int)(threadIdx.x & 63);  // lane within the wavefront
-    const int i = b.xs - 1 + strip * OUT + lane - 1;  // may be -1 or > xmax
-    const bool owner = lane >= 1 && lane <= OUT && i <= xmax;
+    const int i = b.xs - 1 + strip * OW + lane - L0;  // may be < 0 or > xmax
+    const bool owner = lane >= L0 && lane < L0 + OW && i <= xmax;
     const int ci = min(max(i, 0), xmax);                 // clamped load column
     const int jlo = b.ys - 1 + band * R;
     const int jhi = min(jlo + R, ymax + 1);
@@ -133,7 +169,13 @@ __device__ __forceinline__ void tile_f64(const StepIn<double> &f,
     tm[2] = ld32(f.tmask, rowoff(jlo + 1) + ci);
     T ssh_c = new_ssh(1, 0, 1);  // T(i, jlo)
 
-    for (int j = jlo; j < jhi; j++) {
+    // short bands: the row loop is unrolled, so the window shifts at its
+    // end become register renaming instead of moves
+    constexpr int UNROLL = R <= 8 ? R : 1;
+#pragma unroll UNROLL
+    for (int jj = 0; jj < R; jj++) {
+        const int j = jlo + jj;
+        if (j >= jhi) break;  // wavefront-uniform
         load3(j + 1, 2);
         load2(j + 1, 1);
         tm[3] = ld32(f.tmask, rowoff(j + 2) + ci);
@@ -323,7 +365,9 @@ __device__ __forceinline__ float ssh_increment(const ContVals<float> &v, float r
 
 // One mixed-precision tile: as tile_f64 over FP32 arrays, the new ssh as a
 // compensated (hi, lo) pair accumulated in FP64.
-template <int R>
+// Owned columns as in tile_f64: lanes L0 .. L0+OW-1, L0 = (64 - OW) / 2
+// (OW = 48 starts every strip's FP32 stores on a 64-byte boundary).
+template <int R, int OW = OUT>
 __device__ __forceinline__ void tile_mixed(const MixedIn &f,
                                            const StepScalars<float> &p,
                                            const MixedOut &o, int W,
@@ -331,11 +375,13 @@ __device__ __forceinline__ void tile_mixed(const MixedIn &f,
                                            const Bounds &b, int strip, int band,
                                            Ssh tide)
 {
+    constexpr int L0 = (64 - OW) / 2;
+    static_assert(L0 >= 1 && L0 + OW + 2 <= 64, "one neighbour lane below, two above");
     using T = float;
     const int xmax = b.xe + 1, ymax = b.ye + 1;
     const int lane = (int)(threadIdx.x & 63);  // lane within the wavefront
-    const int i = b.xs - 1 + strip * OUT + lane - 1;
-    const bool owner = lane >= 1 && lane <= OUT && i <= xmax;
+    const int i = b.xs - 1 + strip * OW + lane - L0;
+    const bool owner = lane >= L0 && lane < L0 + OW && i <= xmax;
     const int ci = min(max(i, 0), xmax);
     const int jlo = b.ys - 1 + band * R;
     const int jhi = min(jlo + R, ymax + 1);
@@ -395,8 +441,20 @@ __device__ __forceinline__ void tile_mixed(const MixedIn &f,
     tm[1] = ld32(f.tmask, rowoff(jlo) + ci);
     tm[2] = ld32(f.tmask, rowoff(jlo + 1) + ci);
     Ssh ssh_c = new_ssh(1, 0, 1, ld32(f.ssh_lo, rowoff(jlo) + ci));
+    // neighbour values of window slots that the previous row already shifted
+    // (slot k of row j is slot k+1 of row j-1): carried, not shifted again
+    int c_tm0e = from_above(tm[0]), c_tm1e = from_above(tm[1]), c_tm1w = from_below(tm[1]);
+    T c_un1w = from_below(un[1]), c_vn0e = from_above(vn[0]), c_e1v0e = from_above(e1v[0]);
+    T c_hv0e = from_above(hv[0]), c_sv0e = from_above(sv[0]), c_e2u1w = from_below(e2u[1]);
+    T c_hu1w = from_below(hu[1]), c_su1w = from_below(su[1]);
 
-    for (int j = jlo; j < jhi; j++) {
+    // short bands: the row loop is unrolled, so the window shifts at its
+    // end become register renaming instead of moves
+    constexpr int UNROLL = R <= 8 ? R : 1;
+#pragma unroll UNROLL
+    for (int jj = 0; jj < R; jj++) {
+        const int j = jlo + jj;
+        if (j >= jhi) break;  // wavefront-uniform
         load3(j + 1, 2);
         load2(j + 1, 1);
         st_lo1 = ld32(f.ssh_lo, rowoff(j + 1) + ci);
@@ -412,37 +470,41 @@ __device__ __forceinline__ void tile_mixed(const MixedIn &f,
             st32(o.ssh_hi, c, ssh_c.hi);
             st32(o.ssh_lo, c, ssh_c.lo);
         }
-        const int tm_e0 = from_above(tm[1]);
-        const int tm_e1 = from_above(tm[2]);
+        const int tm_e0 = c_tm1e;
+        const int tm_e1 = from_above(tm[2]), tm2w = from_below(tm[2]);
+        const T vn1e = from_above(vn[1]), e1v1e = from_above(e1v[1]);
+        const T hv1e = from_above(hv[1]), sv1e = from_above(sv[1]);
+        const T un2w = from_below(un[2]), e2u2w = from_below(e2u[2]);
+        const T hu2w = from_below(hu[2]), su2w = from_below(su[2]);
         {
             MomUVals<T> v;
             v.m_c = tm[1];
             v.m_e = tm_e0;
             v.m_s = tm[0];
-            v.m_se = from_above(tm[0]);
+            v.m_se = c_tm0e;
             v.m_n = tm[2];
             v.m_ne = tm_e1;
             v.un_c = un[1];
             v.un_e = from_above(un[1]);
-            v.un_w = from_below(un[1]);
+            v.un_w = c_un1w;
             v.un_s = un[0];
             v.un_n = un[2];
             v.vn_s = vn[0];
-            v.vn_se = from_above(vn[0]);
+            v.vn_se = c_vn0e;
             v.vn_c = vn[1];
-            v.vn_e = from_above(vn[1]);
+            v.vn_e = vn1e;
             v.e1v_s = e1v[0];
-            v.e1v_se = from_above(e1v[0]);
+            v.e1v_se = c_e1v0e;
             v.e1v_c = e1v[1];
-            v.e1v_e = from_above(e1v[1]);
+            v.e1v_e = e1v1e;
             v.hv_s = hv[0];
             v.sv_s = sv[0];
-            v.hv_se = from_above(hv[0]);
-            v.sv_se = from_above(sv[0]);
+            v.hv_se = c_hv0e;
+            v.sv_se = c_sv0e;
             v.hv_c = hv[1];
             v.sv_c = sv[1];
-            v.hv_e = from_above(hv[1]);
-            v.sv_e = from_above(sv[1]);
+            v.hv_e = hv1e;
+            v.sv_e = sv1e;
             v.e2t_c = e2t[0];
             v.e2t_e = from_above(e2t[0]);
             v.e1t_c = e1t[0];
@@ -479,26 +541,26 @@ __device__ __forceinline__ void tile_mixed(const MixedIn &f,
             v.m_c = tm[1];
             v.m_e = tm_e0;
             v.m_n = tm[2];
-            v.m_w = from_below(tm[1]);
-            v.m_nw = from_below(tm[2]);
+            v.m_w = c_tm1w;
+            v.m_nw = tm2w;
             v.m_ne = tm_e1;
             v.vn_c = vn[1];
             v.vn_n = vn[2];
             v.vn_s = vn[0];
             v.vn_w = from_below(vn[1]);
-            v.vn_e = from_above(vn[1]);
-            v.un_w = from_below(un[1]);
-            v.un_nw = from_below(un[2]);
+            v.vn_e = vn1e;
+            v.un_w = c_un1w;
+            v.un_nw = un2w;
             v.un_c = un[1];
             v.un_n = un[2];
-            v.e2u_w = from_below(e2u[1]);
-            v.e2u_nw = from_below(e2u[2]);
+            v.e2u_w = c_e2u1w;
+            v.e2u_nw = e2u2w;
             v.e2u_c = e2u[1];
             v.e2u_n = e2u[2];
-            v.hu_w = from_below(hu[1]);
-            v.su_w = from_below(su[1]);
-            v.hu_nw = from_below(hu[2]);
-            v.su_nw = from_below(su[2]);
+            v.hu_w = c_hu1w;
+            v.su_w = c_su1w;
+            v.hu_nw = hu2w;
+            v.su_nw = su2w;
             v.hu_c = hu[1];
             v.su_c = su[1];
             v.hu_n = hu[2];
@@ -513,13 +575,13 @@ __device__ __forceinline__ void tile_mixed(const MixedIn &f,
             v.st_n = st[1];
             v.e1v_c = e1v[1];
             v.e1v_w = from_below(e1v[1]);
-            v.e1v_e = from_above(e1v[1]);
+            v.e1v_e = e1v1e;
             v.hv_c = hv[1];
             v.sv_c = sv[1];
             v.hv_w = from_below(hv[1]);
             v.sv_w = from_below(sv[1]);
-            v.hv_e = from_above(hv[1]);
-            v.sv_e = from_above(sv[1]);
+            v.hv_e = hv1e;
+            v.sv_e = sv1e;
             v.e2v_c = dyv;
             v.fcor = fcv;
             v.area = arv;
@@ -558,6 +620,17 @@ __device__ __forceinline__ void tile_mixed(const MixedIn &f,
         tm[1] = tm[2];
         tm[2] = tm[3];
         ssh_c = ssh_n;
+        c_tm0e = tm_e0;
+        c_tm1e = tm_e1;
+        c_tm1w = tm2w;
+        c_un1w = un2w;
+        c_vn0e = vn1e;
+        c_e1v0e = e1v1e;
+        c_hv0e = hv1e;
+        c_sv0e = sv1e;
+        c_e2u1w = e2u2w;
+        c_hu1w = hu2w;
+        c_su1w = su2w;
     }
 }
 

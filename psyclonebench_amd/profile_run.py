@@ -35,6 +35,9 @@ def main() -> None:
                    help="steps of the FP32 mixed form (plain launches)")
     p.add_argument("--mixed-rows", type=str, default="64",
                    help="rows per band of the mixed kernel, comma separated")
+    p.add_argument("--mixed-cols", type=str, default="61",
+                   help="output columns per wavefront of the mixed kernel (61, 48), "
+                        "comma separated")
     p.add_argument("--closure-train", type=int, default=0,
                    help="closure training steps on synthetic coarse-grid pairs")
     a = p.parse_args()
@@ -71,11 +74,12 @@ def main() -> None:
     if a.mixed:
         from .psy import MixedStepper
         out["mixed_kernel_ms"] = {}
-        for rows in [int(r) for r in a.mixed_rows.split(",")]:
-            st = MixedStepper(Model(cfg), rows=rows)
-            st.run(1)
-            out["mixed_kernel_ms"][rows] = st.timed_run(a.mixed)
-            st.release()
+        for cols in [int(c) for c in a.mixed_cols.split(",")]:
+            for rows in [int(r) for r in a.mixed_rows.split(",")]:
+                st = MixedStepper(Model(cfg), rows=rows, cols=cols)
+                st.run(1)
+                out["mixed_kernel_ms"][f"{rows}x{cols}"] = st.timed_run(a.mixed)
+                st.release()
         out["mixed_alg_bytes_per_launch"] = 113 * (cfg.jpiglo - 2) * (cfg.jpjglo - 2)
     if a.closure_train:
         import numpy as np

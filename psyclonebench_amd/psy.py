@@ -124,7 +124,7 @@ class MixedStepper:
 
     FIELDS = {"sshn_t": 0, "un": 1, "vn": 2, "sshn_u": 3, "sshn_v": 4}
 
-    def __init__(self, model: Model, rows: int = 32):
+    def __init__(self, model: Model, rows: int = 32, cols: int = 61):
         lib = hip_lib()
         g = model.grid
         c = model.cfg
@@ -134,6 +134,7 @@ class MixedStepper:
                                          g.W, g.total, *model.scalars, c.tide_amp,
                                          c.tide_omega)
         lib.nemo_mp_set_rows(self.handle, rows)
+        lib.nemo_mp_set_columns(self.handle, cols)
 
     def run(self, nsteps: int) -> None:
         hip_lib().nemo_mp_steps(self.handle, nsteps, self.istp)

@@ -40,7 +40,7 @@ def _release_contexts():
 
 def _variants():
     # -1 = tile kernel, 0.. = row-march shapes (count fixed by the library)
-    return list(range(-1, 21))
+    return list(range(-1, 31))
 
 
 def test_gpu_visible():
