@@ -3,17 +3,18 @@
 // nemo_mixed.hip) and the ensemble kernels (nemo_ensemble.hip).
 //
 // Overlapped-wavefront march. One block = one wavefront of 64 lanes over 64
-// consecutive columns, of which lanes 1..61 own their column's outputs and
-// lanes 0, 62, 63 are halo (neighbouring strips overlap by 3 columns). Every
-// value a lane needs at column i-1 or i+1 sits in lane l-1 / l+1 of the same
-// wavefront, so it is taken with a cross-lane permute (ds_bpermute) rather
-// than loaded or kept in a second register: each lane carries only its own
-// column's window, which roughly halves the registers of the other march and
-// raises the wavefronts per SIMD. The new ssh of T(i+1) comes from lane l+1,
-// which computes it for its own column (lane 62 is the last one that needs
-// it, and its stencil ends at lane 63).
+// consecutive columns, of which OW lanes own their column's outputs (by
+// default lanes 1..61, OW = 61) and the others are halo (neighbouring strips
+// overlap by 64 - OW columns). Every value a lane needs at column i-1 or i+1
+// sits in lane l-1 / l+1 of the same wavefront, so it is taken with a DPP
+// wave shift rather than loaded or kept in a second register: each lane
+// carries only its own column's window, which roughly halves the registers
+// of the other march and raises the wavefronts per SIMD. The new ssh of
+// T(i+1) comes from lane l+1, which computes it for its own column (the
+// lane after the last owner is the last one that needs it, and its stencil
+// ends one lane further).
 //
-// A tile is one strip (61 output columns, 64 lanes) of one band (R rows) of
+// A tile is one strip (OW output columns, 64 lanes) of one band (R rows) of
 // one domain; the caller picks the tile and passes the arrays, the element
 // offset `base` of the domain's point (0,0) in them, its pitch W and its
 // internal bounds. Element offsets times the element size must stay below
