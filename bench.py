@@ -267,12 +267,13 @@ def ensemble_leg(a) -> dict:
             "unit": "grid-pt updates/s",
             "interior_points": pts,
             "ms_per_step": wall * 1e3 / a.ens_steps,
-            "shape": {"rows": shape[0], "waves_per_workgroup": shape[1],
-                      "workgroups": shape[2]},
+            "shape": {"rows": shape[0], "columns": shape[3],
+                      "waves_per_workgroup": shape[1], "workgroups": shape[2]},
             "autotune_kernel_ms": [round(x, 4) for x in tune_ms],
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                         "kernel": f"k_ens_{'f64' if prec == 64 else 'mixed'}<{shape[0]}>",
+                         "kernel": (f"k_ens_{'f64' if prec == 64 else 'mixed'}"
+                                    f"<{shape[0]}, {shape[3]}>"),
                          "kernel_ms": kernel_ms, "bytes_per_launch_alg": b_alg * pts},
         }
         ens.release()

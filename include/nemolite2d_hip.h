@@ -209,11 +209,13 @@ NEMO_API void *nemo_ens_create(int ndom, void *const *arrays, const int *geom,
                                const double *tide_amp, const double *tide_omega,
                                double rdt, double cbfr, double visc, double omega,
                                double d2r, double g, int precision);
-/* Launch shapes: rows per band (tile height; tile width is 61 columns),
- * wavefronts per workgroup, workgroups per launch. */
+/* Launch shapes: rows per band (tile height), wavefronts per workgroup,
+ * workgroups per launch; nemo_ens_shape_columns gives the tile width (owned
+ * columns per wavefront: 61, or 56 for 64-byte-aligned stores). */
 NEMO_API int nemo_ens_shape_count(void *handle);
 NEMO_API void nemo_ens_shape_info(void *handle, int shape, int *rows,
                                   int *waves_per_block, int *blocks);
+NEMO_API int nemo_ens_shape_columns(void *handle, int shape);
 NEMO_API void nemo_ens_set_shape(void *handle, int shape);
 NEMO_API int nemo_ens_shape(void *handle);
 /* Times every shape on steps_per_shape steps of this ensemble (HIP events

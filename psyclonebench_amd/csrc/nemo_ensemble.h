@@ -26,9 +26,10 @@ struct EnsTile {
 
 // Launch shape of the persistent step kernel.
 struct EnsShape {
-    int rows;              // rows per band (tile height): 16, 32 or 64
+    int rows;              // rows per band (tile height): 4, 8, 16, 32 or 64
     int waves_per_block;   // wavefronts per workgroup: 1, 2 or 4
     int blocks;            // workgroups in the launch (a multiple of 8)
+    int cols;              // owned columns per wavefront (tile width): 61 or 56
 };
 
 // FP64 step of every domain: state `in` -> `out` (arena base pointers).

@@ -2,7 +2,8 @@
 // advances every domain of the ensemble.
 //
 // Work list: every domain is cut into tiles of one overlapped-wavefront strip
-// (61 output columns) by one band (R rows); the list holds (domain, strip,
+// (OW output columns: 61, or 56 so that every strip's stores start on a
+// 64-byte boundary) by one band (R rows); the list holds (domain, strip,
 // band) per tile, domain by domain. The launch has a fixed number of
 // workgroups (a multiple of 8) of 1, 2 or 4 wavefronts; every wavefront takes
 // tiles from the list with a stride and runs wave::tile_f64 / tile_mixed on
@@ -37,7 +38,7 @@ __device__ __forceinline__ Walk walk(int ntiles)
     return Walk{t0 + k * wpb + wave, min(t0 + chunk, ntiles), per * wpb};
 }
 
-template <int R>
+template <int R, int OW>
 __global__ __launch_bounds__(256) void k_ens_f64(StepIn<double> f, StepScalars<double> p,
                                                  StepOut<double> o,
                                                  const EnsDomain *__restrict__ dom,
@@ -49,12 +50,12 @@ __global__ __launch_bounds__(256) void k_ens_f64(StepIn<double> f, StepScalars<d
     for (int t = w.first; t < w.last; t += w.stride) {
         const EnsTile tl = tiles[t];
         const EnsDomain d = dom[tl.dom];
-        wave::tile_f64<R>(f, p, o, d.W, d.off, Bounds{d.xs, d.xe, d.ys, d.ye},
-                          tl.strip, tl.band, tide[tl.dom]);
+        wave::tile_f64<R, OW>(f, p, o, d.W, d.off, Bounds{d.xs, d.xe, d.ys, d.ye},
+                              tl.strip, tl.band, tide[tl.dom]);
     }
 }
 
-template <int R>
+template <int R, int OW>
 __global__ __launch_bounds__(256) void k_ens_mixed(MixedIn f, StepScalars<float> p,
                                                    MixedOut o,
                                                    const EnsDomain *__restrict__ dom,
@@ -66,8 +67,8 @@ __global__ __launch_bounds__(256) void k_ens_mixed(MixedIn f, StepScalars<float>
     for (int t = w.first; t < w.last; t += w.stride) {
         const EnsTile tl = tiles[t];
         const EnsDomain d = dom[tl.dom];
-        wave::tile_mixed<R>(f, p, o, d.W, d.off, Bounds{d.xs, d.xe, d.ys, d.ye},
-                            tl.strip, tl.band, wave::split(tide[tl.dom]));
+        wave::tile_mixed<R, OW>(f, p, o, d.W, d.off, Bounds{d.xs, d.xe, d.ys, d.ye},
+                                tl.strip, tl.band, wave::split(tide[tl.dom]));
     }
 }
 
@@ -78,7 +79,13 @@ void check_shape(const EnsShape &sh)
                    sh.blocks);
     if (sh.waves_per_block != 1 && sh.waves_per_block != 2 && sh.waves_per_block != 4)
         NEMO_FATAL("ensemble launch: %d wavefronts per workgroup", sh.waves_per_block);
+    if (sh.cols != 61 && sh.cols != 56)
+        NEMO_FATAL("ensemble launch: %d columns per wavefront", sh.cols);
 }
+
+// rows x columns of the tile -> one kernel instantiation
+#define NEMO_ENS_SHAPES(X) \
+    X(4, 61) X(8, 61) X(16, 61) X(32, 61) X(64, 61) X(4, 56) X(8, 56) X(16, 56) X(32, 56) X(64, 56)
 
 }  // namespace
 
@@ -89,13 +96,15 @@ void launch_ens_f64(hipStream_t s, const EnsShape &sh, const StepIn<double> &in,
 {
     check_shape(sh);
     const dim3 grid((unsigned)sh.blocks), block((unsigned)(64 * sh.waves_per_block));
-    switch (sh.rows) {
-    case 16: k_ens_f64<16><<<grid, block, 0, s>>>(in, p, out, dom, tiles, ntiles, tide); break;
-    case 32: k_ens_f64<32><<<grid, block, 0, s>>>(in, p, out, dom, tiles, ntiles, tide); break;
-    case 64: k_ens_f64<64><<<grid, block, 0, s>>>(in, p, out, dom, tiles, ntiles, tide); break;
-    default: NEMO_FATAL("ensemble launch: unsupported rows per band %d", sh.rows);
+#define NEMO_ENS_F64(R, OW)                                                                \
+    if (sh.rows == R && sh.cols == OW) {                                                   \
+        k_ens_f64<R, OW><<<grid, block, 0, s>>>(in, p, out, dom, tiles, ntiles, tide);    \
+        NEMO_LAUNCH_CHECK();                                                               \
+        return;                                                                            \
     }
-    NEMO_LAUNCH_CHECK();
+    NEMO_ENS_SHAPES(NEMO_ENS_F64)
+#undef NEMO_ENS_F64
+    NEMO_FATAL("ensemble launch: unsupported tile %d rows x %d columns", sh.rows, sh.cols);
 }
 
 void launch_ens_mixed(hipStream_t s, const EnsShape &sh, const MixedIn &in,
@@ -105,13 +114,15 @@ void launch_ens_mixed(hipStream_t s, const EnsShape &sh, const MixedIn &in,
 {
     check_shape(sh);
     const dim3 grid((unsigned)sh.blocks), block((unsigned)(64 * sh.waves_per_block));
-    switch (sh.rows) {
-    case 16: k_ens_mixed<16><<<grid, block, 0, s>>>(in, p, out, dom, tiles, ntiles, tide); break;
-    case 32: k_ens_mixed<32><<<grid, block, 0, s>>>(in, p, out, dom, tiles, ntiles, tide); break;
-    case 64: k_ens_mixed<64><<<grid, block, 0, s>>>(in, p, out, dom, tiles, ntiles, tide); break;
-    default: NEMO_FATAL("ensemble launch: unsupported rows per band %d", sh.rows);
+#define NEMO_ENS_MIXED(R, OW)                                                              \
+    if (sh.rows == R && sh.cols == OW) {                                                   \
+        k_ens_mixed<R, OW><<<grid, block, 0, s>>>(in, p, out, dom, tiles, ntiles, tide);  \
+        NEMO_LAUNCH_CHECK();                                                               \
+        return;                                                                            \
     }
-    NEMO_LAUNCH_CHECK();
+    NEMO_ENS_SHAPES(NEMO_ENS_MIXED)
+#undef NEMO_ENS_MIXED
+    NEMO_FATAL("ensemble launch: unsupported tile %d rows x %d columns", sh.rows, sh.cols);
 }
 
 int ens_max_waves_per_cu(int precision)
@@ -119,10 +130,10 @@ int ens_max_waves_per_cu(int precision)
     int blocks = 0;
     if (precision == 64)
         NEMO_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &blocks, reinterpret_cast<const void *>(&k_ens_f64<32>), 64, 0));
+            &blocks, reinterpret_cast<const void *>(&k_ens_f64<32, 61>), 64, 0));
     else
         NEMO_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &blocks, reinterpret_cast<const void *>(&k_ens_mixed<32>), 64, 0));
+            &blocks, reinterpret_cast<const void *>(&k_ens_mixed<32, 61>), 64, 0));
     return blocks;
 }
 

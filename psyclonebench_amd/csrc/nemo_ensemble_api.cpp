@@ -10,8 +10,8 @@
 // each domain); precision 32 runs the mixed form (FP32 fields, compensated
 // ssh, FP64 tide), bitwise the single-domain mixed form of each domain.
 //
-// Launch shapes (rows per band x wavefronts per workgroup x resident
-// wavefronts per CU) are enumerated at creation; nemo_ens_autotune times each
+// Launch shapes (rows per band x columns per wavefront x wavefronts per
+// workgroup x resident wavefronts per CU) are enumerated at creation; nemo_ens_autotune times each
 // on the ensemble itself with HIP events and keeps the fastest, restoring the
 // state it advanced.
 #include <cmath>
@@ -31,6 +31,9 @@ namespace nemo {
 namespace {
 
 constexpr int kProg = 6;  // prognostic arenas per set: t (hi), t lo, u, v, su, sv
+// tile shapes of the launch list: rows per band x owned columns per wavefront
+constexpr int kEnsRows[] = {4, 8, 16, 32, 64};
+constexpr int kEnsCols[] = {61, 56};
 
 struct EnsContext {
     int prec = 64;
@@ -54,7 +57,7 @@ struct EnsContext {
     bool paired = false;
     std::vector<EnsShape> shapes;
     int shape = 0;
-    std::map<int, std::pair<EnsTile *, int>> lists;  // rows -> work list
+    std::map<std::pair<int, int>, std::pair<EnsTile *, int>> lists;  // (rows, cols) -> list
     double *d_tide = nullptr, *h_tide = nullptr;
     int tide_cap = 0;                                 // steps
     std::map<std::tuple<int, int, int, int>, hipGraphExec_t> graphs;  // n, parity, shape, row0
@@ -216,33 +219,37 @@ void build_shapes(EnsContext &C)
     const int occ = ens_max_waves_per_cu(C.prec);  // resident waves per CU
     if (occ < 1) NEMO_FATAL("ensemble step kernel does not fit on a CU");
     const int levels[3] = {std::max(1, occ / 3), std::max(1, (2 * occ) / 3), occ};
-    for (int rows : {16, 32, 64})
-        for (int wpb : {1, 2, 4})
-            for (int lv : levels) {
-                const int per_cu = std::max(1, lv / wpb);
-                int blocks = cus * per_cu;
-                blocks = (blocks + 7) / 8 * 8;
-                C.shapes.push_back(EnsShape{rows, wpb, blocks});
-            }
+    for (int cols : kEnsCols)
+        for (int rows : kEnsRows)
+            for (int wpb : {1, 2, 4})
+                for (int lv : levels) {
+                    const int per_cu = std::max(1, lv / wpb);
+                    int blocks = cus * per_cu;
+                    blocks = (blocks + 7) / 8 * 8;
+                    C.shapes.push_back(EnsShape{rows, wpb, blocks, cols});
+                }
 }
 
 void build_lists(EnsContext &C, hipStream_t s)
 {
-    for (int rows : {16, 32, 64}) {
-        std::vector<EnsTile> L;
-        for (int d = 0; d < C.ndom; d++) {
-            const Geometry &G = C.geo[d];
-            const Bounds b{G.xs, G.xe, G.ys, G.ye};
-            const int ns = wave::strips(b), nb = wave::bands(b, rows);
-            for (int band = 0; band < nb; band++)
-                for (int strip = 0; strip < ns; strip++)
-                    L.push_back(EnsTile{d, strip, band, 0});
+    std::vector<std::vector<EnsTile>> host;  // alive until the copies are done
+    for (int cols : kEnsCols)
+        for (int rows : kEnsRows) {
+            std::vector<EnsTile> L;
+            for (int d = 0; d < C.ndom; d++) {
+                const Geometry &G = C.geo[d];
+                const Bounds b{G.xs, G.xe, G.ys, G.ye};
+                const int ns = (b.xe - b.xs + 3 + cols - 1) / cols, nb = wave::bands(b, rows);
+                for (int band = 0; band < nb; band++)
+                    for (int strip = 0; strip < ns; strip++)
+                        L.push_back(EnsTile{d, strip, band, 0});
+            }
+            EnsTile *dl = dev_alloc<EnsTile>(L.size());
+            NEMO_HIP_CHECK(hipMemcpyAsync(dl, L.data(), L.size() * sizeof(EnsTile),
+                                          hipMemcpyHostToDevice, s));
+            C.lists[{rows, cols}] = {dl, (int)L.size()};
+            host.push_back(std::move(L));
         }
-        EnsTile *dl = dev_alloc<EnsTile>(L.size());
-        NEMO_HIP_CHECK(hipMemcpyAsync(dl, L.data(), L.size() * sizeof(EnsTile),
-                                      hipMemcpyHostToDevice, s));
-        C.lists[rows] = {dl, (int)L.size()};
-    }
     NEMO_HIP_CHECK(hipStreamSynchronize(s));
 }
 
@@ -303,7 +310,7 @@ void launch_pair(EnsContext &C, int par, int row, hipStream_t s, hipEvent_t e0 =
 {
     const int q = 1 - par;
     const EnsShape &sh = C.shapes[C.shape];
-    const auto &L = C.lists.at(sh.rows);
+    const auto &L = C.lists.at({sh.rows, sh.cols});
     const double *tide = C.d_tide + (size_t)row * C.ndom;
     if (e0) NEMO_HIP_CHECK(hipEventRecord(e0, s));
     if (C.prec == 64) {
@@ -493,9 +500,11 @@ void *nemo_ens_create(int ndom, void *const *arrays, const int *geom,
     build_flather_lists(*C, arrays, s);
     build_lists(*C, s);
     build_shapes(*C);
-    // default: 32 rows, 1 wavefront per workgroup, full residency
+    // default: 32 rows x 61 columns, 1 wavefront per workgroup, full residency
     for (size_t k = 0; k < C->shapes.size(); k++)
-        if (C->shapes[k].rows == 32 && C->shapes[k].waves_per_block == 1) C->shape = (int)k;
+        if (C->shapes[k].rows == 32 && C->shapes[k].cols == 61 &&
+            C->shapes[k].waves_per_block == 1)
+            C->shape = (int)k;
     NEMO_HIP_CHECK(hipStreamSynchronize(s));
     return C.release();
 }
@@ -521,6 +530,14 @@ void nemo_ens_set_shape(void *h, int shape)
 }
 
 int nemo_ens_shape(void *h) { return ctx(h).shape; }
+
+int nemo_ens_shape_columns(void *h, int shape)
+{
+    EnsContext &C = ctx(h);
+    if (shape < 0 || shape >= (int)C.shapes.size())
+        NEMO_FATAL("unknown ensemble shape %d", shape);
+    return C.shapes[shape].cols;
+}
 
 void nemo_ens_steps(void *h, int nsteps, int istep) { run(ctx(h), nsteps, istep, false); }
 

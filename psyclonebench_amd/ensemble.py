@@ -84,14 +84,15 @@ class EnsembleStepper:
     def interior_points(self) -> int:
         return int(hip_lib().nemo_ens_interior_points(self.handle))
 
-    def shapes(self) -> list[tuple[int, int, int]]:
-        """(rows per band, wavefronts per workgroup, workgroups) per shape."""
+    def shapes(self) -> list[tuple[int, int, int, int]]:
+        """(rows per band, wavefronts per workgroup, workgroups, columns per
+        wavefront) per shape."""
         lib = hip_lib()
         out = []
         for k in range(lib.nemo_ens_shape_count(self.handle)):
             r, w, b = C.c_int(), C.c_int(), C.c_int()
             lib.nemo_ens_shape_info(self.handle, k, C.byref(r), C.byref(w), C.byref(b))
-            out.append((r.value, w.value, b.value))
+            out.append((r.value, w.value, b.value, lib.nemo_ens_shape_columns(self.handle, k)))
         return out
 
     @property

@@ -60,17 +60,21 @@ def test_ensemble_f64_matches_oracle_per_domain():
     ens.release()
 
 
-def test_ensemble_every_shape_bitwise_equal():
+@pytest.mark.parametrize("precision", [64, 32])
+def test_ensemble_every_shape_bitwise_equal(precision):
+    """Every launch shape (rows x columns x wavefronts per workgroup x
+    residency) gives the same bits, in both precisions."""
     cfgs = small_ensemble(ndom=5, seed=7)
     models = [Model(c) for c in cfgs]
-    probe = EnsembleStepper(models, precision=64)
+    probe = EnsembleStepper(models, precision=precision)
     nshape = len(probe.shapes())
+    assert {s[3] for s in probe.shapes()} == {61, 56}
     probe.release()
     ref = None
     for k in range(nshape):
-        ens = EnsembleStepper(models, precision=64)
+        ens = EnsembleStepper(models, precision=precision)
         ens.set_shape(k)
-        print("shape", k, ens.shapes()[k], flush=True)
+        print("precision", precision, "shape", k, ens.shapes()[k], flush=True)
         ens.run(12)
         out = [{n: ens.fetch(d, n) for n in ("sshn_t", "un", "vn")}
                for d in range(len(cfgs))]
