@@ -375,7 +375,7 @@ def main():
             "hbm_frac_whole_step": B_ALG_F64 * updates_per_s / (HBM_PEAK_GBS * 1e9),
             "fused_variant": best,
             "fused_shape": {"block_columns": bw.value, "rows": rows.value,
-                            "kind": (f"overlapped-wavefront march, {56 if pf.value == 5 else 48} "
+                            "kind": (f"overlapped-wavefront march, {48 if pf.value == 6 else 56} "
                                      "columns, 64-byte-aligned stores" if pf.value >= 5 else
                                      ["row march", "row march, loads one row ahead",
                                       "overlapped-wavefront march"][min(pf.value, 2)])},

@@ -239,7 +239,7 @@ void build_lists(EnsContext &C, hipStream_t s)
             for (int d = 0; d < C.ndom; d++) {
                 const Geometry &G = C.geo[d];
                 const Bounds b{G.xs, G.xe, G.ys, G.ye};
-                const int ns = (b.xe - b.xs + 3 + cols - 1) / cols, nb = wave::bands(b, rows);
+                const int ns = wave::strips(b, cols), nb = wave::bands(b, rows);
                 for (int band = 0; band < nb; band++)
                     for (int strip = 0; strip < ns; strip++)
                         L.push_back(EnsTile{d, strip, band, 0});

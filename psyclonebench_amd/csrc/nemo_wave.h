@@ -73,7 +73,10 @@ __device__ __forceinline__ double from_above(double x)
 
 // Strips and bands of the extent the fused step walks: columns xs-1 .. xe+1,
 // rows ys-1 .. ye+1.
-__host__ __device__ inline int strips(const Bounds &b) { return (b.xe - b.xs + 3 + OUT - 1) / OUT; }
+__host__ __device__ inline int strips(const Bounds &b, int ow)
+{
+    return (b.xe - b.xs + 3 + ow - 1) / ow;
+}
 __host__ __device__ inline int bands(const Bounds &b, int R) { return (b.ye - b.ys + 3 + R - 1) / R; }
 
 // Block b of a launch of nblk blocks -> logical tile, so that each of the 8
