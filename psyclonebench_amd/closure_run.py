@@ -21,7 +21,11 @@ back toward the fine solution. A rollout whose step loss exceeds
 --reset-spike times the seed's running mean is reset at once: a state that
 far off would only feed the network gradients of its own drift. The saved
 weights are a moving average of the weights over the second half of the
-run (--ema decay per sample, one sample every --ema-every steps). The
+run (--ema decay per sample, one sample every --ema-every steps). Every
+--val-every seeds the current weights run --val-steps coarse steps from rest
+on a held-out bathymetry (--val-bathy) beside the plain coarse model, and the
+weights with the lowest error ratio (plus the fraction of corrections the
+energy guard rejected) are the ones saved. The
 loss and the correction leave out the ssh of cells the step clamps to the
 tide, and an activity gate 1 - exp(-(a / --gate-eps)^2), a = |u| + |v| +
 |ssh| in input units, scales the output so that water at rest gets no
@@ -164,7 +168,9 @@ def train(a) -> None:
     spin_rng = np.random.default_rng(a.seed + 1)
     ema = None                    # moving average of the weights (--ema)
     spikes = 0
-    for b in bathy:
+    best = None                   # (score, params, after seed) of the validation
+    validations = []
+    for bi, b in enumerate(bathy):
         fine, coarse = pair(a.n, a.factor, b, a.steps)
         sf, sc = TimeStepper(fine, mode=2), TimeStepper(coarse, mode=2)
         # the training window of this seed starts after a random spin-up of
@@ -230,20 +236,58 @@ def train(a) -> None:
         sc.release()
         params = net.params() if ema is None else ema.astype(np.float32)
         save(a.out, params, norm, a, history)
+        if a.val_every > 0 and ((bi + 1) % a.val_every == 0 or bi == len(bathy) - 1):
+            score, ratios, trips = validation_score(a, params, norm, history)
+            validations.append({"after_seed": b, "score": score, "ratio_mean": ratios,
+                                "guard_trips": trips})
+            log(f"validation after seed {b}: score {score:.3f}  ratios "
+                f"{ {k: round(v, 3) for k, v in ratios.items()} }  trips {trips}")
+            if best is None or score < best[0]:
+                best = (score, params.copy(), b)
     net.release()
+    if best is not None:
+        params = best[1]
+        save(a.out, params, norm, a, history,
+             extra={"validation": validations, "selected_after_seed": best[2]})
+        if best[0] >= 1.0:
+            log("warning: no checkpoint beat the plain coarse model on the validation "
+                f"bathymetry (best score {best[0]:.3f}); the saved network is not an "
+                "improvement")
     log(f"trained {done} steps in {time.perf_counter() - t0:.0f} s -> {a.out}"
         + (f" (moving average of the weights, decay {a.ema})" if ema is not None else "")
-        + (f", {spikes} rollouts restarted on a loss spike" if a.reset_spike > 0 else ""))
+        + (f", {spikes} rollouts restarted on a loss spike" if a.reset_spike > 0 else "")
+        + (f"; kept the weights after seed {best[2]} (validation score {best[0]:.3f})"
+           if best is not None else ""))
 
 
-def save(path, params, norm, a, history) -> None:
+def validation_score(a, params, norm, history) -> tuple[float, dict, int]:
+    """Model selection: the learned / plain error ratio of `params` on the
+    held-out --val-bathy seed, over --val-steps coarse steps from rest (mean
+    over time, averaged over u, v and the free-cell ssh), plus the fraction
+    of those steps whose correction the energy guard rejected."""
+    tmp = Path(a.out).with_name(Path(a.out).stem + ".validation.npz")
+    save(tmp, params, norm, a, history)
+    try:
+        res = evaluate(a.n, a.factor, [a.val_bathy], a.val_steps,
+                       max(a.val_steps // 10, 1), tmp, log_fn=lambda *x: None)
+    finally:
+        tmp.unlink()
+    seed = res["seeds"][a.val_bathy]
+    ratios = {f: float(res["ratio_mean"][f]) for f in ("un", "vn", "sshn_t_free")}
+    trips = int(seed["guard_trips"])
+    return float(np.mean(list(ratios.values()))) + trips / a.val_steps, ratios, trips
+
+
+def save(path, params, norm, a, history, extra=None) -> None:
     meta = {"norm": norm, "n_fine": a.n, "factor": a.factor, "bathy": a.bathy,
             "coarse_steps_per_seed": a.steps, "lr": a.lr, "lr_min": a.lr_min,
             "rollout": a.rollout, "guard_rel": a.guard_rel, "clip": a.clip,
             "max_spin": a.max_spin, "loss_weights": list(a.loss_weights),
             "ema": a.ema, "ema_every": a.ema_every, "ema_start": a.ema_start,
-            "reset_spike": a.reset_spike,
+            "reset_spike": a.reset_spike, "val_bathy": a.val_bathy,
+            "val_every": a.val_every, "val_steps": a.val_steps,
             "optimizer": "adam", "history": history}
+    meta.update(extra or {})
     Path(path).parent.mkdir(parents=True, exist_ok=True)
     np.savez(path, params=np.asarray(params, np.float32), meta=json.dumps(meta))
 
@@ -400,6 +444,13 @@ def main(argv=None) -> None:
     t.add_argument("--reset-spike", type=float, default=4.0,
                    help="restart a rollout from the filtered fine state when a step's loss "
                         "exceeds this multiple of the seed's running mean (0: never)")
+    t.add_argument("--val-bathy", type=int, default=1000,
+                   help="held-out bathymetry seed of the model-selection runs")
+    t.add_argument("--val-every", type=int, default=8,
+                   help="training seeds between model-selection runs (0: keep the last "
+                        "weights)")
+    t.add_argument("--val-steps", type=int, default=1000,
+                   help="coarse steps of each model-selection run")
     t.add_argument("--resume", default=None)
     t.add_argument("--out", default=str(DEFAULT_WEIGHTS))
     e = sub.add_parser("eval")

@@ -239,7 +239,8 @@ def test_shipped_weights_beat_the_plain_coarse_model():
 def test_training_command_end_to_end(tmp_path):
     """The `closure_run train` command at a small size (fine 256^2, coarse
     64^2): pilot normalisation, spin-ups, rollouts with spike restarts, the
-    moving average of the weights, the saved file, and an evaluation of it."""
+    moving average of the weights, model selection on a held-out seed, the
+    saved file, and an evaluation of it."""
     import json as _json
 
     from psyclonebench_amd import closure_run as cr
@@ -247,13 +248,18 @@ def test_training_command_end_to_end(tmp_path):
     print("closure train command: fine 256^2, 2 seeds x 30 steps", flush=True)
     cr.main(["train", "--n", "256", "--bathy", "0:2", "--steps", "30", "--pilot-steps", "100",
              "--max-spin", "5", "--rollout", "5", "--ema", "0.9", "--ema-every", "2",
-             "--ema-start", "0", "--reset-spike", "4", "--log-every", "10", "--out", str(out)])
+             "--ema-start", "0", "--reset-spike", "4", "--log-every", "10",
+             "--val-every", "1", "--val-steps", "40", "--out", str(out)])
     z = np.load(out)
     meta = _json.loads(str(z["meta"]))
     assert z["params"].shape == (_native.hip_lib().nemo_cl_param_count(),)
     assert np.all(np.isfinite(z["params"]))
     assert meta["ema"] == 0.9 and meta["reset_spike"] == 4.0
     assert len(meta["history"]) == 2
+    # model selection ran after each seed and kept one of the two
+    assert [v["after_seed"] for v in meta["validation"]] == [0, 1]
+    assert meta["selected_after_seed"] in (0, 1)
+    assert not (tmp_path / "w.validation.npz").exists()
     res = cr.evaluate(256, 4, [2], 20, 10, out, log_fn=lambda *x: None)
     for f in ("sshn_t", "un", "vn"):
         assert math.isfinite(res["ratio_mean"][f]), f
