@@ -233,3 +233,42 @@ def test_gravity_wave_speed_of_the_discrete_scheme():
     c_eff = math.sqrt(9.80665 * 100.0 / 1000.0)
     speed = (6 - 2) * 1000.0 / ((first[6] - first[2]) * 20.0)
     assert 0.7 * c_eff < speed < 1.5 * c_eff, (speed, c_eff, first)
+
+
+def test_initialise_face_ssh_from_nonzero_ssh_closed_form():
+    """initialisation_mod.f90:35-55 from a nonzero ssh. T-cell areas twice the
+    face areas and an ssh linear along one axis: a face between two cells
+    takes twice their midpoint, a + b (k + 1/2); a face across the other axis
+    sees one value twice, 2 s. A dropped term, a neighbour on the wrong side,
+    a missing area ratio or the two axes swapped breaks it. Only faces away
+    from the clamped last column / row are checked: where the clamp starts is
+    a dl_esm_inf extent that no reference file fixes."""
+    lib = _native.setup_lib()
+    M = N = 12
+    W, H = 16, N + 1
+    at = np.full((H, W), 3.0e6)
+    au = np.full((H, W), 1.5e6)
+    av = np.full((H, W), 1.5e6)
+    a, b = 0.25, 0.01
+    for axis in ("x", "y"):
+        ii = np.arange(W, dtype=float)[None, :] + 0.0 * np.arange(H)[:, None]
+        jj = np.arange(H, dtype=float)[:, None] + 0.0 * np.arange(W)[None, :]
+        s = a + b * (ii if axis == "x" else jj)
+        su = np.zeros((H, W))
+        sv = np.zeros((H, W))
+        un = np.ones((H, W))
+        vn = np.ones((H, W))
+        lib.nemo_initialise(M, N, W, H, _ptr(at), _ptr(au), _ptr(av), _ptr(s), _ptr(su),
+                            _ptr(sv), _ptr(un), _ptr(vn))
+        rows, cols = slice(0, N - 1), slice(0, M - 1)
+        if axis == "x":
+            k = np.arange(M - 3, dtype=float)            # faces i = 0 .. M-4
+            want = np.broadcast_to(2 * (a + b * (k + 0.5))[None, :], (N - 1, M - 3))
+            np.testing.assert_allclose(su[rows, 0:M - 3], want, rtol=1e-14)
+            np.testing.assert_allclose(sv[0:N - 3, cols], 2 * s[0:N - 3, cols], rtol=1e-14)
+        else:
+            k = np.arange(N - 3, dtype=float)            # faces j = 0 .. N-4
+            want = np.broadcast_to(2 * (a + b * (k + 0.5))[:, None], (N - 3, M - 1))
+            np.testing.assert_allclose(sv[0:N - 3, cols], want, rtol=1e-14)
+            np.testing.assert_allclose(su[rows, 0:M - 3], 2 * s[rows, 0:M - 3], rtol=1e-14)
+        assert not np.any(un) and not np.any(vn)
