@@ -298,7 +298,7 @@ def ensemble_leg(a) -> dict:
 def main():
     a = parse_args()
     if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+        print(f"bench.py: the library runs on one GPU; --gpus {a.gpus} is not supported "
               "(the step would shard by 2-D domain decomposition with depth-1 halos; "
               "not built)", file=sys.stderr)
         sys.exit(2)

@@ -7,7 +7,7 @@
 ! shows a Fortran host driving the HIP step through exactly that interface.
 ! It reads the reference's namelist (&namctl with the defaults of
 ! common/gocean2d_io_mod.f90:58-69), sets the model up through
-! libnemo_setup.so (dl_esm_inf and model_mod are not part of this build),
+! libnemo_setup.so (dl_esm_inf and model_mod are not in this repository),
 ! steps nit000..nitend and logs the checksums of the reference driver
 ! (psykal_c_opencl/nemolite2d.f90:116-120).
 module nemo_hip_abi
