@@ -135,6 +135,14 @@ inline void check_geometry(const Geometry &G)
                    G.W, G.ye);
 }
 
+// The fused and mixed kernels address every array with a 32-bit byte offset
+// from a uniform base (ld32 / st32 in nemo_point.h). True when an array of
+// G.total elements of elem_bytes each fits that addressing.
+inline bool offsets_fit_32(const Geometry &G, size_t elem_bytes)
+{
+    return (unsigned long long)G.total * elem_bytes <= (1ull << 32);
+}
+
 // Coriolis factor (2*omega)*sin(gphi*d2r) per point, host libm, the same
 // expression as momentum_u_kern.c:251 / momentum_v_kern.c:230.
 inline std::vector<double> coriolis(const double *gphi, int n, double omega, double d2r)

@@ -163,6 +163,9 @@ void *nemo_mp_create(NEMO_HOST_ARRAYS, int internal_xstart, int internal_xstop,
     const Geometry G{internal_xstart, internal_xstop, internal_ystart, internal_ystop,
                      width, total_size};
     check_geometry(G);
+    if (!offsets_fit_32(G, sizeof(float)))
+        NEMO_FATAL("mixed form: total_size %d x 4 bytes exceeds the 4 GiB its kernels "
+                   "address with 32-bit offsets", total_size);
     if (!fused_allowed(tmask, G))
         NEMO_FATAL("mixed form: the mask has wet cells on row %d", G.ye + 1);
     const std::vector<signed char> tm8 = mask_bytes(tmask, (size_t)total_size);

@@ -153,6 +153,17 @@ std::unique_ptr<Context> make_context(double *const host[A_COUNT],
                                       int mode_request)
 {
     check_geometry(G);
+    // The fused form's mask condition and its 32-bit addressing are checked
+    // before any device call.
+    const int *tm = reinterpret_cast<const int *>(host[A_TMASK]);
+    const bool addr_ok = offsets_fit_32(G, sizeof(double));
+    if (mode_request == 2 && !addr_ok)
+        NEMO_FATAL("fused form requested but total_size %d x 8 bytes exceeds the 4 GiB "
+                   "its kernels address with 32-bit offsets", G.total);
+    const bool mask_ok = mode_request != 1 && addr_ok && fused_allowed(tm, G);
+    if (mode_request == 2 && !mask_ok)
+        NEMO_FATAL("fused form requested but the mask has wet cells on row %d",
+                   G.ye + 1);
     auto C = std::make_unique<Context>();
     C->G = G;
     C->P = P;
@@ -165,12 +176,7 @@ std::unique_ptr<Context> make_context(double *const host[A_COUNT],
     hipStream_t s = lib_stream();
     upload_all(*C, host, s);
 
-    const int *tm = reinterpret_cast<const int *>(host[A_TMASK]);
-    const bool ok = fused_allowed(tm, G);
-    if (mode_request == 2 && !ok)
-        NEMO_FATAL("fused form requested but the mask has wet cells on row %d",
-                   G.ye + 1);
-    C->mode = (mode_request == 1 || !ok) ? 1 : 2;
+    C->mode = (mode_request == 1 || !mask_ok || !addr_ok) ? 1 : 2;
     if (C->mode == 2) {
         C->spare_su = dev_alloc<double>(n);
         C->spare_sv = dev_alloc<double>(n);
@@ -360,6 +366,46 @@ using namespace nemo;
      area_v_device, dx_u_device, dx_v_device, dx_t_device, dy_u_device,        \
      dy_v_device, dy_t_device, gphiu_device, gphiv_device}
 
+static bool same_geometry(const Geometry &a, const Geometry &b)
+{
+    return a.W == b.W && a.total == b.total && a.xs == b.xs && a.xe == b.xe &&
+           a.ys == b.ys && a.ye == b.ye;
+}
+
+static bool same_physics(const Physics &a, const Physics &b)
+{
+    return a.rdt == b.rdt && a.cbfr == b.cbfr && a.visc == b.visc && a.omega == b.omega &&
+           a.d2r == b.d2r && a.g == b.g;
+}
+
+// New physical scalars for an existing context: the Coriolis factors and the
+// Flather coefficients derive from them, and captured graphs carry them as
+// kernel arguments. The inputs they derive from (gphiu/v, tmask, hu/hv) are
+// read back from the device, where the caller's last upload or
+// c_write_to_device left them; the step never writes them.
+static void refresh_physics(Context &C, const Physics &P)
+{
+    hipStream_t s = lib_stream();
+    NEMO_HIP_CHECK(hipStreamSynchronize(s));
+    const size_t n = (size_t)C.G.total;
+    std::vector<double> gu(n), gv(n), hu(n), hv(n);
+    std::vector<int> tm(n);
+    NEMO_HIP_CHECK(hipMemcpy(gu.data(), C.buf[A_GPHIU], n * sizeof(double), hipMemcpyDeviceToHost));
+    NEMO_HIP_CHECK(hipMemcpy(gv.data(), C.buf[A_GPHIV], n * sizeof(double), hipMemcpyDeviceToHost));
+    NEMO_HIP_CHECK(hipMemcpy(hu.data(), C.buf[A_HU], n * sizeof(double), hipMemcpyDeviceToHost));
+    NEMO_HIP_CHECK(hipMemcpy(hv.data(), C.buf[A_HV], n * sizeof(double), hipMemcpyDeviceToHost));
+    NEMO_HIP_CHECK(hipMemcpy(tm.data(), C.buf[A_TMASK], n * sizeof(int), hipMemcpyDeviceToHost));
+    C.P = P;
+    const std::vector<double> fu = coriolis(gu.data(), C.G.total, P.omega, P.d2r);
+    const std::vector<double> fv = coriolis(gv.data(), C.G.total, P.omega, P.d2r);
+    NEMO_HIP_CHECK(hipMemcpy(C.fcor_u, fu.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    NEMO_HIP_CHECK(hipMemcpy(C.fcor_v, fv.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    C.fl_u.upload(build_flather(tm.data(), hu.data(), C.G, P.g, true), s);
+    C.fl_v.upload(build_flather(tm.data(), hv.data(), C.G, P.g, false), s);
+    for (auto &kv : C.graphs) NEMO_HIP_CHECK(hipGraphExecDestroy(kv.second));
+    C.graphs.clear();
+}
+
 static Context &device_context(double *const host[A_COUNT], void **slot[A_COUNT],
                                const Geometry &G, const Physics &P)
 {
@@ -367,10 +413,9 @@ static Context &device_context(double *const host[A_COUNT], void **slot[A_COUNT]
     if (it == g_contexts.end())
         it = g_contexts.emplace(slot[A_SSHA_T], make_context(host, G, P, g_mode)).first;
     Context &C = *it->second;
-    if (C.G.W != G.W || C.G.total != G.total || C.G.xs != G.xs || C.G.xe != G.xe ||
-        C.G.ys != G.ys || C.G.ye != G.ye)
+    if (!same_geometry(C.G, G))
         NEMO_FATAL("geometry changed between calls for the same device slots");
-    C.P = P;
+    if (!same_physics(C.P, P)) refresh_physics(C, P);
     return C;
 }
 
@@ -456,17 +501,22 @@ void c_invoke_time_step_host(NEMO_HOST_ARRAYS, int istep, int internal_xstart,
                      internal_ystop, width, total};
     const Physics P{rdt, cbfr, visc, omega, d2r, g};
     hipStream_t s = lib_stream();
+    // A context is reused only for the same geometry: its Flather segment
+    // lists depend on every bound. upload_all rebuilds the derived arrays
+    // from this call's host arrays and scalars.
     auto it = g_host_contexts.find(ssha_t);
-    if (it == g_host_contexts.end() || it->second->G.total != total ||
-        it->second->G.W != width) {
-        if (it != g_host_contexts.end()) g_host_contexts.erase(it);
+    if (it != g_host_contexts.end() && !same_geometry(it->second->G, G)) {
+        NEMO_HIP_CHECK(hipStreamSynchronize(s));
+        g_host_contexts.erase(it);
+        it = g_host_contexts.end();
+    }
+    if (it == g_host_contexts.end()) {
         it = g_host_contexts.emplace(ssha_t, make_context(host, G, P, 1)).first;
     } else {
         it->second->P = P;
         upload_all(*it->second, host, s);
     }
     Context &C = *it->second;
-    C.G = G;
     step_unfused(C, istep, s);
     const size_t bytes = (size_t)total * sizeof(double);
     const Arr outs[] = {A_SSHA_T, A_SSHN_T, A_SSHN_U, A_SSHN_V,

@@ -57,6 +57,43 @@ def kernel_inputs(mask_name: str) -> dict[str, np.ndarray]:
     return {k[len(pre):]: v.copy() for k, v in z.items() if k.startswith(pre)}
 
 
+def live_state_model(n: int, seed: int, random_mask: bool = False) -> Model:
+    """An n^2 model whose 24 FP64 arrays are seeded random and nonzero over
+    the whole allocation (padding included), in the value ranges of the
+    per-kernel fixtures: ssh and velocities in [-1, 1], depths in [50, 150],
+    areas in [5e5, 2e6], lengths in [500, 1500], latitudes in [-80, 80].
+    Exact +-0 velocities exercise the upwind selectors. The mask is the
+    open-west one of the 4096^2 workload, or (random_mask) seeded {-1, 0, 1}
+    with no wet cell on row 0, column 0 or the last T row (which the fused
+    form requires)."""
+    from psyclonebench_amd.model import c2_config
+    m = Model(c2_config(n, 1, seed=0))
+    g = m.grid
+    rng = np.random.default_rng(seed)
+    ranges = dict(ssha_t=(-1, 1), sshn_t=(-1, 1), sshn_u=(-1, 1), sshn_v=(-1, 1),
+                  hu=(50, 150), hv=(50, 150), un=(-1, 1), vn=(-1, 1), ua=(-1, 1),
+                  ht=(50, 150), ssha_u=(-1, 1), va=(-1, 1), ssha_v=(-1, 1),
+                  area_t=(5e5, 2e6), area_u=(5e5, 2e6), area_v=(5e5, 2e6),
+                  dx_u=(500, 1500), dx_v=(500, 1500), dx_t=(500, 1500),
+                  dy_u=(500, 1500), dy_v=(500, 1500), dy_t=(500, 1500),
+                  gphiu=(-80, 80), gphiv=(-80, 80))
+    for name, (lo, hi) in ranges.items():
+        a = m.array(name)
+        a[:] = rng.uniform(lo, hi, size=a.shape)
+    m.array("un")[g.N // 2, 100:400] = 0.0
+    m.array("vn")[g.N // 3, 200:500] = -0.0
+    if random_mask:
+        t = g.tmask
+        t[:] = 0
+        r = rng.choice(np.array([-1, 0, 1], np.int32), size=(g.N, g.M),
+                       p=[0.1, 0.1, 0.8])
+        r[0, :] = np.minimum(r[0, :], 0)
+        r[:, 0] = np.minimum(r[:, 0], 0)
+        r[g.N - 1, :] = np.minimum(r[g.N - 1, :], 0)
+        t[:g.N, :g.M] = r
+    return m
+
+
 def assert_bitwise(a: np.ndarray, b: np.ndarray, what: str = "") -> None:
     """Equal as bit patterns (so -0.0 != 0.0 and NaN payloads count)."""
     assert a.shape == b.shape, (what, a.shape, b.shape)

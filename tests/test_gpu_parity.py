@@ -13,8 +13,9 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import (GOLDEN, assert_bitwise, kernel_fixture, kernel_inputs, load_npz,
-                     manifest, model_for_case, step_case_names)
+from helpers import (GOLDEN, assert_bitwise, kernel_fixture, kernel_inputs,
+                     live_state_model, load_npz, manifest, model_for_case,
+                     step_case_names)
 from psyclonebench_amd import _native
 from psyclonebench_amd.model import (MASK_CLOSED, Model, NemoConfig, c2_config,
                                      reference_config)
@@ -31,11 +32,14 @@ MODE_AUTO, MODE_UNFUSED, MODE_FUSED = 0, 1, 2
 
 @pytest.fixture(autouse=True)
 def _release_contexts():
-    yield
+    # every test starts on the variant the test before it found (the
+    # library default, 26, unless a test sets one) and leaves it that way
     lib = _native.hip_lib()
+    variant = lib.nemo_hip_fused_variant()
+    yield
     lib.nemo_hip_release_all()
     lib.nemo_hip_set_kernel_timing(0)
-    lib.nemo_hip_set_fused_variant(0)
+    lib.nemo_hip_set_fused_variant(variant)
 
 
 def _variants():
@@ -273,60 +277,113 @@ def test_closed_basin_volume_on_gpu():
     assert abs(v1 - v0) / float(np.sum(area[wet])) < 1e-13
 
 
-@pytest.mark.slow
-def test_full_size_4096_fused_equals_oracle():
-    """The 4096^2 workload (seeded bathymetry, west-open tide): 12 fused
-    graph-replayed steps on the GPU vs the OpenMP oracle on the host,
-    bitwise on every prognostic field."""
-    cfg = c2_config(4096, 12, seed=0)
-    m = Model(cfg)
+# ---- full size: 4096^2 with live data everywhere ------------------------
+# After a fused call each after/now pair shares one buffer, so the after
+# fields and the face ssh are compared everywhere and the now fields inside
+# the reference's copy region (rows 0..N-2, columns 0..M-1).
+AFTER_OUT = ("ssha_t", "ua", "va", "sshn_u", "sshn_v")
+NOW_OUT = ("sshn_t", "un", "vn")
+
+
+def _oracle_run(m: Model, nsteps: int) -> dict:
     g = m.grid
     host = [a.copy() for a in m.arrays()]
-    print("4096^2: GPU", flush=True)
-    st = TimeStepper(m, MODE_FUSED)
-    st.run(12)
-    gpu = {n: st.fetch(n) for n in ("sshn_t", "un", "vn", "sshn_u", "sshn_v")}
-    st.release()
-    print("4096^2: oracle", flush=True)
-    oracle.set_tide(cfg.tide_amp, cfg.tide_omega)
+    oracle.set_tide(m.cfg.tide_amp, m.cfg.tide_omega)
     try:
-        oracle.run_steps(host, 1, 12, g.bounds, g.W, m.scalars, kind="oracle_omp")
+        oracle.run_steps(host, 1, nsteps, g.bounds, g.W, m.scalars, kind="oracle_omp")
     finally:
         oracle.oracle_lib().oracle_reset_tide()
-    for n in gpu:
-        ref = host[oracle.ARRAY_NAMES.index(n)]
-        assert_bitwise(gpu[n][:g.N - 1], ref[:g.N - 1], n)
+    return dict(zip(oracle.ARRAY_NAMES, host))
+
+
+def _gpu_run(m: Model, nsteps: int, mode: int, graph: bool = True) -> tuple[dict, int]:
+    st = TimeStepper(m, mode)
+    if graph:
+        st.run(nsteps)
+    else:
+        for _ in range(nsteps):
+            st.step()
+    out = {n: st.fetch(n) for n in AFTER_OUT + NOW_OUT}
+    mode_used = st.context_mode
+    st.release()
+    return out, mode_used
+
+
+def _assert_live_equal(out: dict, ref: dict, g, what: str) -> None:
+    for n in AFTER_OUT:
+        assert_bitwise(out[n], ref[n], f"{what} {n}")
+    for n in NOW_OUT:
+        assert_bitwise(out[n][:g.N - 1, :g.M], ref[n][:g.N - 1, :g.M], f"{what} {n}")
 
 
 @pytest.mark.slow
-def test_full_size_4096_fused_equals_unfused():
-    cfg = c2_config(4096, 1, seed=1)
-    a, b = Model(cfg), Model(cfg)
-    sa, sb = TimeStepper(a, MODE_FUSED), TimeStepper(b, MODE_UNFUSED)
-    sa.run(40)
-    sb.run(40)
-    for n in ("sshn_t", "un", "vn", "sshn_u", "sshn_v"):
-        x, y = sa.fetch(n), sb.fetch(n)
-        assert_bitwise(x[:a.grid.N - 1], y[:a.grid.N - 1], n)
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("random_mask", [False, True], ids=["west_open", "random_mask"])
+def test_full_size_4096_live_state_equals_oracle(random_mask):
+    """4096^2 with every array seeded random and nonzero (state, depths,
+    metrics, latitudes, padding): 3 graph-replayed fused steps on the
+    library's default kernel against the OpenMP oracle, bitwise on all eight
+    outputs. From rest most of a 4096^2 grid stays exactly zero for
+    thousands of steps, where a wrong metric or depth load cannot show."""
+    m = live_state_model(4096, seed=11 + int(random_mask), random_mask=random_mask)
+    g = m.grid
+    print("4096^2 live state, random mask", random_mask, ": oracle", flush=True)
+    ref = _oracle_run(m, 3)
+    print("4096^2 live state: GPU, variant", _native.hip_lib().nemo_hip_fused_variant(),
+          flush=True)
+    out, mode = _gpu_run(m.copy(), 3, MODE_FUSED)
+    assert mode == MODE_FUSED
+    _assert_live_equal(out, ref, g, "fused")
 
 
 @pytest.mark.slow
-def test_full_size_4096_every_variant_equal():
-    """Every fused kernel shape gives the same bits at full size (odd
-    interior 4093 so that no shape divides the extent)."""
-    cfg = c2_config(4093, 1, seed=2)
-    ref = None
+@pytest.mark.timeout(900)
+def test_full_size_live_state_every_variant_equals_oracle():
+    """Every fused kernel shape on a 4093^2 live state (odd extent, so that
+    no strip or band width divides it), 2 steps, bitwise the oracle."""
+    m = live_state_model(4093, seed=21)
+    g = m.grid
+    print("4093^2 live state: oracle", flush=True)
+    ref = _oracle_run(m, 2)
     for v in _variants():
         print("variant", v, flush=True)
         _native.hip_lib().nemo_hip_set_fused_variant(v)
-        m = Model(cfg)
-        st = TimeStepper(m, MODE_FUSED)
-        st.run(6)
-        out = {n: st.fetch(n)[:m.grid.N - 1] for n in ("sshn_t", "un", "vn", "sshn_u",
-                                                      "sshn_v")}
-        st.release()
-        if ref is None:
-            ref = out
-        else:
-            for n in out:
-                assert_bitwise(out[n], ref[n], f"variant {v} {n}")
+        out, mode = _gpu_run(m.copy(), 2, MODE_FUSED)
+        assert mode == MODE_FUSED
+        _assert_live_equal(out, ref, g, f"variant {v}")
+
+
+@pytest.mark.slow
+@pytest.mark.timeout(900)
+def test_full_size_4096_live_state_unfused_equals_oracle():
+    """The unfused in-place sequence (one kernel per reference sweep) on the
+    4096^2 live state, 3 single-step calls, bitwise the oracle everywhere."""
+    m = live_state_model(4096, seed=31)
+    ref = _oracle_run(m, 3)
+    out, mode = _gpu_run(m.copy(), 3, MODE_UNFUSED, graph=False)
+    assert mode == MODE_UNFUSED
+    for n in AFTER_OUT + NOW_OUT:
+        assert_bitwise(out[n], ref[n], f"unfused {n}")
+
+
+@pytest.mark.slow
+@pytest.mark.timeout(1200)
+def test_c2_workload_1000_steps_equals_oracle():
+    """The 4096^2 FP64 workload itself (seeded bathymetry, open west edge,
+    1 m tide) for its full 1000 steps as one graph replay, against 1000
+    OpenMP oracle steps: bitwise on every output. The north-star bound is
+    1e-12 relative; the bar met is equality."""
+    cfg = c2_config(4096, 1000, seed=0)
+    m = Model(cfg)
+    g = m.grid
+    print("C2 1000 steps: GPU", flush=True)
+    out, mode = _gpu_run(m.copy(), 1000, MODE_FUSED)
+    assert mode == MODE_FUSED
+    print("C2 1000 steps: oracle (OpenMP)", flush=True)
+    ref = _oracle_run(m, 1000)
+    _assert_live_equal(out, ref, g, "C2 step 1000")
+    # the signal from the west edge has reached about 180 columns (values
+    # down to 1e-274 at the front): the comparison is not only of zeros
+    live = np.count_nonzero(ref["sshn_t"][g.ys:g.ye + 1, g.xs:g.xe + 1])
+    print("C2 1000 steps: nonzero interior ssh points", live, flush=True)
+    assert live > 150 * (g.N - 2)

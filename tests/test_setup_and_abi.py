@@ -166,6 +166,44 @@ def test_mixed_form_rejects_mask_values_beyond_a_byte():
     assert "T mask value 100" in r.stderr + r.stdout, r.stderr[-2000:]
 
 
+_BIG_GRID_CALL = """
+import ctypes as C, numpy as np
+from psyclonebench_amd import _native
+lib = _native.hip_lib()
+d = np.zeros(8); t = np.zeros(8, np.int32)
+dp = d.ctypes.data_as(C.POINTER(C.c_double)); ip = t.ctypes.data_as(C.POINTER(C.c_int))
+host = [ip if k == 13 else dp for k in range(25)]
+keep = [C.c_void_p() for _ in range(25)]
+slots = [C.cast(C.byref(v), C.POINTER(C.c_void_p)) for v in keep]
+W, ye = {W}, {ye}
+tail = (1, 1, W - 100, 1, ye, W, W * (ye + 2), 20.0, 0.0015, 0.1, 7.292116e-05, 0.0174, 9.80665)
+{call}
+"""
+
+
+@pytest.mark.parametrize("form", ["fused", "mixed"])
+def test_32_bit_offset_limit_is_rejected_before_any_device_call(form):
+    """The fused FP64 and the mixed FP32 kernels address arrays with 32-bit
+    byte offsets: a grid whose arrays exceed 4 GiB (FP64 above about 23k^2,
+    FP32 above about 32k^2) is refused on the host with the library's
+    message and exit(1), before anything touches a device. The host arrays
+    passed here are tiny; a check that read them would crash instead."""
+    import subprocess
+    import sys
+    if form == "fused":
+        code = _BIG_GRID_CALL.format(W=32768, ye=16400, call=(
+            "lib.nemo_hip_set_mode(2)\n"
+            "lib.c_invoke_time_step(*host, *slots, *tail)"))
+    else:
+        code = _BIG_GRID_CALL.format(W=32768, ye=32768, call=(
+            "lib.nemo_mp_create(*host, *tail[1:], 1.0, 1.4e-4)"))
+    root = Path(__file__).resolve().parents[1]
+    r = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 1, (r.returncode, r.stderr[-2000:])
+    assert "32-bit offsets" in r.stderr, r.stderr[-2000:]
+
+
 @pytest.mark.parametrize("call", ["c_read_from_device(None, None, 1, 1, -3, 4, True)",
                                   "c_write_to_device(None, None, 1, 1, 8, 8, True)"])
 def test_copy_callbacks_reject_bad_arguments(call):
