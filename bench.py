@@ -16,7 +16,9 @@ Prints ONE JSON line (rank 0, N = 1):
                    over its mean duration from HIP events, against 8 TB/s
   cpu_baseline     the reference's own OpenMP PSy layer (time_step_omp.cpp,
                    built into oracle/_ref) on this host's cores, on a bounded
-                   sample of the same workload
+                   sample of the same workload; "rows" adds its serial layer
+                   (time_step.cpp) pinned to one core, and both at 128^2 for
+                   2000 steps; "host" names the CPU (lscpu)
 """
 from __future__ import annotations
 
@@ -77,40 +79,45 @@ def traffic_from_profile(label: str):
     return None
 
 
-def cpu_baseline(n: int, seconds: float) -> dict:
-    """The reference OpenMP PSy layer (oracle/_ref/libref_time_step_omp.so)
-    timed on this host on k steps of the same n^2 workload, k chosen so the
-    sample takes about `seconds`."""
-    import numpy as np  # noqa: F401
-    import oracle
-    from psyclonebench_amd.model import Model, c2_config
+def _cpu_row(args: list[str]) -> dict:
+    """One CPU timing (oracle/cpu_bench.py) in a process of its own, so that
+    the serial row's core pinning and the OpenMP settings stay apart."""
+    import subprocess
+    r = subprocess.run([sys.executable, "-m", "oracle.cpu_bench", *args], cwd=ROOT,
+                       capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError(f"cpu_bench {args} failed: {r.stderr[-2000:]}")
+    return json.loads(r.stdout.strip().splitlines()[-1])
 
-    affinity = len(os.sched_getaffinity(0))
-    threads = min(affinity, int(os.environ.get("OMP_NUM_THREADS", affinity)))
-    os.environ["OMP_NUM_THREADS"] = str(threads)
-    kind = "ref_omp" if oracle.ref_available() else "oracle_omp"
-    m = Model(c2_config(n, 1, seed=0))
-    g = m.grid
-    arrays = m.arrays()
-    step = oracle.step_fn(kind)
-    t0 = time.perf_counter()
-    step(arrays, 1, g.bounds, g.W, m.scalars)
-    one = time.perf_counter() - t0
-    k = max(2, min(200, int(seconds / max(one, 1e-6))))
-    t0 = time.perf_counter()
-    for istp in range(2, 2 + k):
-        step(arrays, istp, g.bounds, g.W, m.scalars)
-    dt = time.perf_counter() - t0
-    pts = (g.M - 2) * (g.N - 2)
+
+def cpu_baseline(n: int, seconds: float) -> dict:
+    """The reference's CPU PSy layer on this host, in four rows: OpenMP
+    (time_step_omp.cpp, the reported value) and serial (time_step.cpp,
+    pinned to one core) on a bounded sample of the n^2 workload, then both
+    on the 128^2 configuration for its full 2000 steps. Each row's process
+    records the host CPU (lscpu) and the cores it used."""
+    omp = _cpu_row(["--n", str(n), "--mode", "omp", "--seconds", str(seconds)])
+    serial = _cpu_row(["--n", str(n), "--mode", "serial", "--seconds", str(seconds)])
+    c1_serial = _cpu_row(["--mode", "serial", "--c1", "--steps", "2000"])
+    c1_omp = _cpu_row(["--mode", "omp", "--c1", "--steps", "2000"])
+    rows = []
+    for name, r in ((f"{n}^2 OpenMP", omp), (f"{n}^2 serial", serial),
+                    ("128^2 x 2000 steps serial", c1_serial),
+                    ("128^2 x 2000 steps OpenMP", c1_omp)):
+        rows.append({"row": name, "value": r["value"], "cores": r["cores"],
+                     "steps_timed": r["steps_timed"], "seconds": r["seconds"],
+                     "code": r["code"], "pinning": r["pinning"]})
     return {
-        "value": pts * k / dt,
+        "value": omp["value"],
         "unit": "grid-pt updates/s",
-        "cores": threads,
-        "kind": "reference" if kind == "ref_omp" else "port",
-        "sample": (f"{k} steps of the {n}x{n} FP64 workload after 1 untimed step; "
-                   f"reference manual_versions/psykal_cpp/time_step_omp.cpp, "
-                   f"OpenMP {threads} threads, g++ -O3 -mtune=native -ffp-contract=off"),
-        "seconds": dt,
+        "cores": omp["cores"],
+        "kind": omp["kind"],
+        "sample": (f"{omp['steps_timed']} steps of the {n}x{n} FP64 workload after 1 "
+                   f"untimed step; reference manual_versions/psykal_cpp/time_step_omp.cpp, "
+                   f"OpenMP {omp['cores']} threads ({omp['pinning']}), {omp['flags']}"),
+        "seconds": omp["seconds"],
+        "host": omp["host"],
+        "rows": rows,
     }
 
 
@@ -375,7 +382,9 @@ def main():
             "hbm_frac_whole_step": B_ALG_F64 * updates_per_s / (HBM_PEAK_GBS * 1e9),
             "fused_variant": best,
             "fused_shape": {"block_columns": bw.value, "rows": rows.value,
-                            "kind": (f"overlapped-wavefront march, {48 if pf.value == 6 else 56} "
+                            "kind": ("overlapped-wavefront march, 56 columns, 64-byte-aligned "
+                                     "stores, loads two rows ahead" if pf.value >= 7 else
+                                     f"overlapped-wavefront march, {48 if pf.value == 6 else 56} "
                                      "columns, 64-byte-aligned stores" if pf.value >= 5 else
                                      ["row march", "row march, loads one row ahead",
                                       "overlapped-wavefront march"][min(pf.value, 2)])},

@@ -154,7 +154,8 @@ NEMO_API int nemo_hip_fused_variant(void);
  * 2/3/4 = overlapped-wavefront march with at least 1/3/4 waves per SIMD
  * requested from the register allocator, 5 / 6 = overlapped-wavefront
  * march with 56 / 48 output columns per wavefront (64-byte-aligned
- * stores). */
+ * stores), 7 / 8 = the 56-column march with its loads issued two rows
+ * ahead, compiled for at least 2 / 3 waves per SIMD. */
 NEMO_API void nemo_hip_fused_variant_shape(int variant, int *block_width,
                                            int *rows, int *prefetch);
 /* Name of the kernel a variant launches, as profilers report it. */

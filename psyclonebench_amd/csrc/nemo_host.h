@@ -31,6 +31,10 @@ struct Physics {
 // The library's one stream (default priority, non-blocking w.r.t. stream 0).
 hipStream_t lib_stream();
 
+// bc_ssh elevation of step istep with the tide set by nemo_hip_set_tide
+// (host libm, as boundary_conditions_kern.c:187-192).
+double tide_at(int istep, double rdt);
+
 template <typename T>
 inline T *dev_alloc(size_t n)
 {

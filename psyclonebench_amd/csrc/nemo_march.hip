@@ -462,6 +462,32 @@ __global__ __launch_bounds__(64) void k_fused_wave_a(
                           tide_tab ? tide_tab[tide_k] : tide_val);
 }
 
+// The aligned march with the loads issued two rows ahead (tile_f64's
+// PF = 1); MINW wavefronts per SIMD requested from the register allocator.
+template <int R, int OW, int MINW>
+__global__ __launch_bounds__(64, MINW) void k_fused_wave_p(
+    StepIn<double> f, StepScalars<double> p, StepOut<double> o, int W, Bounds b,
+    int nstrip, const double *__restrict__ tide_tab, int tide_k, double tide_val)
+{
+    static_assert(OW % 8 == 0, "64-byte-aligned strips");
+    const int logical = wave::xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const int band = logical / nstrip, strip = logical - band * nstrip;
+    wave::tile_f64<R, OW, 1>(f, p, o, W, 0u, b, strip, band,
+                             tide_tab ? tide_tab[tide_k] : tide_val);
+}
+
+template <int R, int OW, int MINW>
+void launch_wave_prefetch_t(hipStream_t s, const StepIn<double> &f,
+                            const StepScalars<double> &p, const StepOut<double> &o, int W,
+                            Bounds b, const double *tide_tab, int tide_k, double tide_val)
+{
+    const int ncol = b.xe - b.xs + 3, nrow = b.ye - b.ys + 3;
+    const int nstrip = (ncol + OW - 1) / OW, nband = (nrow + R - 1) / R;
+    k_fused_wave_p<R, OW, MINW><<<dim3((unsigned)(nstrip * nband)), dim3(64), 0, s>>>(
+        f, p, o, W, b, nstrip, tide_tab, tide_k, tide_val);
+    NEMO_LAUNCH_CHECK();
+}
+
 template <int R, int OW>
 void launch_wave_aligned_t(hipStream_t s, const StepIn<double> &f,
                            const StepScalars<double> &p, const StepOut<double> &o, int W,
@@ -491,7 +517,8 @@ void launch_wave_t(hipStream_t s, const StepIn<double> &f,
 // 3 / 4 the overlapped-wavefront march compiled for >= 3 / 4 wavefronts per
 // SIMD (the compiler keeps its registers within that budget), 5 / 6 the
 // overlapped-wavefront march with 56 / 48 output columns (64-byte-aligned
-// stores; with 48 the loads are aligned too)
+// stores; with 48 the loads are aligned too), 7 / 8 the 56-column march with
+// its loads issued two rows ahead, compiled for >= 2 / 3 wavefronts per SIMD
 struct MarchShape {
     int mx, rows, prefetch;
 };
@@ -519,7 +546,8 @@ static const MarchShape kShapes[] = {
     {64, 64, 2},  {64, 128, 2}, {64, 32, 3},  {64, 64, 3},  {64, 32, 4},
     {64, 64, 4},  {64, 16, 5},  {64, 32, 5},  {64, 8, 5},   {64, 16, 6},
     {64, 32, 6},  {64, 4, 5},   {64, 8, 6},   {64, 4, 6},   {64, 2, 5},
-    {64, 6, 5},
+    {64, 6, 5},   {64, 4, 7},   {64, 6, 7},   {64, 8, 7},   {64, 12, 7},
+    {64, 4, 8},   {64, 8, 8},
 };
 
 // Kernel names as the profiler reports them (template arguments included).
@@ -539,7 +567,10 @@ static const char *const kNames[] = {
     "k_fused_wave_a<16, 48>",        "k_fused_wave_a<32, 48>",
     "k_fused_wave_a<4, 56>",         "k_fused_wave_a<8, 48>",
     "k_fused_wave_a<4, 48>",         "k_fused_wave_a<2, 56>",
-    "k_fused_wave_a<6, 56>",
+    "k_fused_wave_a<6, 56>",         "k_fused_wave_p<4, 56, 2>",
+    "k_fused_wave_p<6, 56, 2>",      "k_fused_wave_p<8, 56, 2>",
+    "k_fused_wave_p<12, 56, 2>",     "k_fused_wave_p<4, 56, 3>",
+    "k_fused_wave_p<8, 56, 3>",
 };
 static_assert(sizeof(kNames) / sizeof(kNames[0]) == sizeof(kShapes) / sizeof(kShapes[0]),
               "one kernel name per shape");
@@ -622,6 +653,24 @@ void launch_fused_march(hipStream_t s, int variant, const StepIn<double> &f,
         break;
     case 30:
         launch_wave_aligned_t<6, 56>(s, f, p, o, W, b, tide_tab, tide_k, tide_val);
+        break;
+    case 31:
+        launch_wave_prefetch_t<4, 56, 2>(s, f, p, o, W, b, tide_tab, tide_k, tide_val);
+        break;
+    case 32:
+        launch_wave_prefetch_t<6, 56, 2>(s, f, p, o, W, b, tide_tab, tide_k, tide_val);
+        break;
+    case 33:
+        launch_wave_prefetch_t<8, 56, 2>(s, f, p, o, W, b, tide_tab, tide_k, tide_val);
+        break;
+    case 34:
+        launch_wave_prefetch_t<12, 56, 2>(s, f, p, o, W, b, tide_tab, tide_k, tide_val);
+        break;
+    case 35:
+        launch_wave_prefetch_t<4, 56, 3>(s, f, p, o, W, b, tide_tab, tide_k, tide_val);
+        break;
+    case 36:
+        launch_wave_prefetch_t<8, 56, 3>(s, f, p, o, W, b, tide_tab, tide_k, tide_val);
         break;
     default: NEMO_FATAL("unknown fused march variant %d", variant);
     }

@@ -39,6 +39,8 @@ int g_fused_variant = 26;
 bool g_kernel_timing = false;
 double g_fused_ms = -1.0;
 
+}  // namespace
+
 // bc_ssh value for one step: amp * sin(omega_tide * (istep*rdt)), evaluated
 // on the host with libm exactly as boundary_conditions_kern.c:187-192 does.
 double tide_at(int istep, double rdt)
@@ -46,6 +48,8 @@ double tide_at(int istep, double rdt)
     const double rtime = istep * rdt;
     return g_amp_tide * std::sin(g_omega_tide * rtime);
 }
+
+namespace {
 
 
 // One model's device state.

@@ -96,7 +96,16 @@ __device__ __forceinline__ int xcd_tile(int bid, int nblk)
 // column needs the new ssh of the column after it, hence two lanes above).
 // OW = 56 with L0 = 4 starts every strip's stores on a 64-byte boundary
 // (56 doubles = 448 bytes), at the price of 8 shared columns instead of 3.
-template <int R, int OW = OUT>
+//
+// PF = 0: the loads of row j+1 are issued at the top of row j, which needs
+// them, so a wavefront waits for them and only other wavefronts overlap
+// that wait. PF = 1: the window holds one more row and the loads of row j+2
+// (and the row-j+1 values of the per-point arrays) are issued at the top of
+// row j, two rows before use, behind a scheduling barrier that keeps the
+// compiler from moving them next to their use; the band loop is then fully
+// unrolled so that the window shifts are register renaming, never copies of
+// values still in flight.
+template <int R, int OW = OUT, int PF = 0>
 __device__ __forceinline__ void tile_f64(const StepIn<double> &f,
                                          const StepScalars<double> &p,
                                          const StepOut<double> &o, int W,
@@ -106,6 +115,8 @@ __device__ __forceinline__ void tile_f64(const StepIn<double> &f,
 {
     constexpr int L0 = (64 - OW) / 2;
     static_assert(L0 >= 1 && L0 + OW + 2 <= 64, "one neighbour lane below, two above");
+    static_assert(PF == 0 || PF == 1, "prefetch depth 0 or 1");
+    static_assert(PF == 0 || R <= 16, "the prefetching band loop is fully unrolled");
     using T = double;
     const int xmax = b.xe + 1, ymax = b.ye + 1;
     const int lane = (int)(threadIdx.x & 63);  // lane within the wavefront
@@ -117,10 +128,14 @@ __device__ __forceinline__ void tile_f64(const StepIn<double> &f,
     auto rowoff = [&](int j) { return base + (unsigned)(min(max(j, 0), ymax) * W); };
 
     // own-column window: 3-slot rows j-1, j, j+1; 2-slot rows j, j+1;
-    // tm rows j-1 .. j+2
-    T un[3], vn[3], hu[3], su[3], hv[3], sv[3], e1v[3], e2u[3];
-    T ht[2], st[2], e1t[2], e2t[2], at[2];
-    int tm[4];
+    // tm rows j-1 .. j+2; with PF = 1 one more row of each in flight, and
+    // the per-point values of rows j and j+1
+    T un[3 + PF], vn[3 + PF], hu[3 + PF], su[3 + PF], hv[3 + PF], sv[3 + PF],
+        e1v[3 + PF], e2u[3 + PF];
+    T ht[2 + PF], st[2 + PF], e1t[2 + PF], e2t[2 + PF], at[2 + PF];
+    int tm[4 + PF];
+    T dxu[1 + PF], dyv[1 + PF], fcu[1 + PF], fcv[1 + PF], aru[1 + PF], arv[1 + PF],
+        sau[1 + PF], sav[1 + PF];
 
     auto load3 = [&](int j, int s) {
         const unsigned c = rowoff(j) + ci;
@@ -164,6 +179,19 @@ __device__ __forceinline__ void tile_f64(const StepIn<double> &f,
         return tide_applies(tm[t], tm[t - 1], tm[t + 1], m_e, m_w) ? tide : x;
     };
 
+    // per-point values of row r into slot s
+    auto load1 = [&](int r, int s) {
+        const unsigned c = rowoff(r) + ci;
+        dxu[s] = ld32(f.dx_u, c);
+        dyv[s] = ld32(f.dy_v, c);
+        fcu[s] = ld32(f.fcor_u, c);
+        fcv[s] = ld32(f.fcor_v, c);
+        aru[s] = ld32(f.area_u, c);
+        arv[s] = ld32(f.area_v, c);
+        sau[s] = ld32(f.ssha_u, c);
+        sav[s] = ld32(f.ssha_v, c);
+    };
+
     // ---- prologue ----
     load3(jlo - 1, 0);
     load3(jlo, 1);
@@ -171,21 +199,38 @@ __device__ __forceinline__ void tile_f64(const StepIn<double> &f,
     tm[0] = ld32(f.tmask, rowoff(jlo - 1) + ci);
     tm[1] = ld32(f.tmask, rowoff(jlo) + ci);
     tm[2] = ld32(f.tmask, rowoff(jlo + 1) + ci);
+    if constexpr (PF == 1) {
+        load3(jlo + 1, 2);
+        load2(jlo + 1, 1);
+        tm[3] = ld32(f.tmask, rowoff(jlo + 2) + ci);
+        load1(jlo, 0);
+    }
     T ssh_c = new_ssh(1, 0, 1);  // T(i, jlo)
 
     // short bands: the row loop is unrolled, so the window shifts at its
     // end become register renaming instead of moves
-    constexpr int UNROLL = R <= 8 ? R : 1;
+    constexpr int UNROLL = (PF == 1 || R <= 8) ? R : 1;
 #pragma unroll UNROLL
     for (int jj = 0; jj < R; jj++) {
         const int j = jlo + jj;
         if (j >= jhi) break;  // wavefront-uniform
-        load3(j + 1, 2);
-        load2(j + 1, 1);
-        tm[3] = ld32(f.tmask, rowoff(j + 2) + ci);
+        if constexpr (PF == 1) {
+            if (jj + 1 < R) {  // rows the band still needs, two ahead
+                load3(j + 2, 3);
+                load2(j + 2, 2);
+                tm[4] = ld32(f.tmask, rowoff(j + 3) + ci);
+                load1(j + 1, 1);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        } else {
+            load3(j + 1, 2);
+            load2(j + 1, 1);
+            tm[3] = ld32(f.tmask, rowoff(j + 2) + ci);
+            load1(j, 0);
+        }
         const unsigned c = rowoff(j) + ci;
-        const T dxu = ld32(f.dx_u, c), dyv = ld32(f.dy_v, c), fcu = ld32(f.fcor_u, c), fcv = ld32(f.fcor_v, c);
-        const T aru = ld32(f.area_u, c), arv = ld32(f.area_v, c), sau = ld32(f.ssha_u, c), sav = ld32(f.ssha_v, c);
+        const T dxu0 = dxu[0], dyv0 = dyv[0], fcu0 = fcu[0], fcv0 = fcv[0];
+        const T aru0 = aru[0], arv0 = arv[0], sau0 = sau[0], sav0 = sav[0];
 
         const T ssh_n = new_ssh(2, 1, 2);   // T(i, j+1)
         const T ssh_e = from_above(ssh_c);  // T(i+1, j)
@@ -245,10 +290,10 @@ __device__ __forceinline__ void tile_f64(const StepIn<double> &f,
             v.e2u_c = e2u[1];
             v.e2u_s = e2u[0];
             v.e2u_n = e2u[2];
-            v.e1u_c = dxu;
-            v.fcor = fcu;
-            v.area = aru;
-            v.ssha = sau;
+            v.e1u_c = dxu0;
+            v.fcor = fcu0;
+            v.area = aru0;
+            v.ssha = sau0;
             T val = T(0);
             bool wrote = in_rect(i, j, b.xs, b.xe - 1, b.ys, b.ye) &&
                          momentum_u_val(v, p, &val);
@@ -305,10 +350,10 @@ __device__ __forceinline__ void tile_f64(const StepIn<double> &f,
             v.sv_w = from_below(sv[1]);
             v.hv_e = from_above(hv[1]);
             v.sv_e = from_above(sv[1]);
-            v.e2v_c = dyv;
-            v.fcor = fcv;
-            v.area = arv;
-            v.ssha = sav;
+            v.e2v_c = dyv0;
+            v.fcor = fcv0;
+            v.area = arv0;
+            v.ssha = sav0;
             T val = T(0);
             bool wrote = in_rect(i, j, b.xs, b.xe, b.ys, b.ye - 1) &&
                          momentum_v_val(v, p, &val);
@@ -323,25 +368,24 @@ __device__ __forceinline__ void tile_f64(const StepIn<double> &f,
             const T at_e0 = from_above(at[0]);
             T val;
             if (owner && in_rect(i, j, b.xs, b.xe - 1, b.ys, b.ye) &&
-                next_ssh_pt(ssh_c, ssh_e, at[0], at_e0, aru, tm[1], tm_e0, &val))
+                next_ssh_pt(ssh_c, ssh_e, at[0], at_e0, aru0, tm[1], tm_e0, &val))
                 st32(o.sshu, c, val);
             if (owner && in_rect(i, j, b.xs, b.xe, b.ys, b.ye - 1) &&
-                next_ssh_pt(ssh_c, ssh_n, at[0], at[1], arv, tm[1], tm[2], &val))
+                next_ssh_pt(ssh_c, ssh_n, at[0], at[1], arv0, tm[1], tm[2], &val))
                 st32(o.sshv, c, val);
         }
 
-#define NEMO_SHIFT3(a) \
-    a[0] = a[1];       \
-    a[1] = a[2];
-#define NEMO_SHIFT2(a) a[0] = a[1];
-        NEMO_SHIFT3(un) NEMO_SHIFT3(vn) NEMO_SHIFT3(hu) NEMO_SHIFT3(su)
-        NEMO_SHIFT3(hv) NEMO_SHIFT3(sv) NEMO_SHIFT3(e1v) NEMO_SHIFT3(e2u)
-        NEMO_SHIFT2(ht) NEMO_SHIFT2(st) NEMO_SHIFT2(e1t) NEMO_SHIFT2(e2t) NEMO_SHIFT2(at)
-#undef NEMO_SHIFT3
-#undef NEMO_SHIFT2
-        tm[0] = tm[1];
-        tm[1] = tm[2];
-        tm[2] = tm[3];
+#define NEMO_SHIFT(a, n)                    \
+    _Pragma("unroll") for (int k = 0; k + 1 < (n); k++) a[k] = a[k + 1];
+        NEMO_SHIFT(un, 3 + PF) NEMO_SHIFT(vn, 3 + PF) NEMO_SHIFT(hu, 3 + PF)
+        NEMO_SHIFT(su, 3 + PF) NEMO_SHIFT(hv, 3 + PF) NEMO_SHIFT(sv, 3 + PF)
+        NEMO_SHIFT(e1v, 3 + PF) NEMO_SHIFT(e2u, 3 + PF)
+        NEMO_SHIFT(ht, 2 + PF) NEMO_SHIFT(st, 2 + PF) NEMO_SHIFT(e1t, 2 + PF)
+        NEMO_SHIFT(e2t, 2 + PF) NEMO_SHIFT(at, 2 + PF) NEMO_SHIFT(tm, 4 + PF)
+        NEMO_SHIFT(dxu, 1 + PF) NEMO_SHIFT(dyv, 1 + PF) NEMO_SHIFT(fcu, 1 + PF)
+        NEMO_SHIFT(fcv, 1 + PF) NEMO_SHIFT(aru, 1 + PF) NEMO_SHIFT(arv, 1 + PF)
+        NEMO_SHIFT(sau, 1 + PF) NEMO_SHIFT(sav, 1 + PF)
+#undef NEMO_SHIFT
         ssh_c = ssh_n;
     }
 }

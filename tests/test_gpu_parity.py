@@ -44,7 +44,7 @@ def _release_contexts():
 
 def _variants():
     # -1 = tile kernel, 0.. = row-march shapes (count fixed by the library)
-    return list(range(-1, 31))
+    return list(range(-1, 37))
 
 
 def test_gpu_visible():
