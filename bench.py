@@ -38,7 +38,9 @@ METRIC = ("grid-pt updates/s + % HBM peak at 4096²; "
 HBM_PEAK_GBS = 8000.0
 B_ALG_F64 = 212          # bytes per interior grid-point update, FP64
 B_ALG_MIXED = 113        # FP32 fields, the byte mask and the ssh compensation word
-PROFILE_TRAFFIC = ROOT / "profiles" / "step_kernels_4096_traffic.md"
+# rocprofv3 summaries holding "[label] traffic_bytes_per_launch:" lines
+PROFILE_TRAFFIC = (ROOT / "profiles" / "fused_step_4096_counters.md",
+                   ROOT / "profiles" / "step_kernels_4096_traffic.md")
 
 
 def parse_args():
@@ -70,12 +72,13 @@ def traffic_from_profile(label: str):
     "mixed rows 64"), measured with rocprofv3 PMC counters (FETCH_SIZE +
     WRITE_SIZE, scaled by the copy-kernel calibration) and kept in
     profiles/; None when that kernel shape was not profiled."""
-    if not PROFILE_TRAFFIC.exists():
-        return None
     key = f"[{label}] traffic_bytes_per_launch:"
-    for line in PROFILE_TRAFFIC.read_text().splitlines():
-        if line.strip().startswith(key):
-            return float(line.split(":", 1)[1])
+    for path in PROFILE_TRAFFIC:
+        if not path.exists():
+            continue
+        for line in path.read_text().splitlines():
+            if line.strip().startswith(key):
+                return float(line.split(":", 1)[1])
     return None
 
 
@@ -382,7 +385,10 @@ def main():
             "hbm_frac_whole_step": B_ALG_F64 * updates_per_s / (HBM_PEAK_GBS * 1e9),
             "fused_variant": best,
             "fused_shape": {"block_columns": bw.value, "rows": rows.value,
-                            "kind": ("overlapped-wavefront march, 56 columns, 64-byte-aligned "
+                            "kind": ("persistent chunked march, 56 columns, 64-byte-aligned stores"
+                                     + (", loads two rows ahead" if pf.value == 9 else "")
+                                     if pf.value >= 9 else
+                                     "overlapped-wavefront march, 56 columns, 64-byte-aligned "
                                      "stores, loads two rows ahead" if pf.value >= 7 else
                                      f"overlapped-wavefront march, {48 if pf.value == 6 else 56} "
                                      "columns, 64-byte-aligned stores" if pf.value >= 5 else

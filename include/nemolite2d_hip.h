@@ -144,7 +144,8 @@ NEMO_API double nemo_hip_fused_kernel_ms(void);
 /* Kernel shape of the fused form: -1 = 64x4-cell tiles with an LDS ssh
  * tile; 0 .. nemo_hip_fused_variant_count()-1 = row-march shapes (block
  * width x rows per band). All shapes compute bitwise the same result;
- * default 26 (4-row bands of 56 columns, the fastest measured at 4096^2).
+ * default 31 (4-row bands of 56 columns with the loads issued two rows
+ * ahead, the fastest measured at 4096^2).
  * Applies to the next launches of every context. */
 NEMO_API int nemo_hip_fused_variant_count(void);
 NEMO_API void nemo_hip_set_fused_variant(int variant);
@@ -155,7 +156,9 @@ NEMO_API int nemo_hip_fused_variant(void);
  * requested from the register allocator, 5 / 6 = overlapped-wavefront
  * march with 56 / 48 output columns per wavefront (64-byte-aligned
  * stores), 7 / 8 = the 56-column march with its loads issued two rows
- * ahead, compiled for at least 2 / 3 waves per SIMD. */
+ * ahead, compiled for at least 2 / 3 waves per SIMD, 9 / 10 = the persistent
+ * chunked 56-column march with / without the loads two rows ahead (rows = 0:
+ * one chunk of rows per wavefront the GPU holds at once, set at launch). */
 NEMO_API void nemo_hip_fused_variant_shape(int variant, int *block_width,
                                            int *rows, int *prefetch);
 /* Name of the kernel a variant launches, as profilers report it. */

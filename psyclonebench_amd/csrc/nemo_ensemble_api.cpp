@@ -268,6 +268,7 @@ void build_flather_lists(EnsContext &C, void *const *arrays, hipStream_t s)
             for (size_t k = 1; k < L.seg_ptr.size(); k++) A.seg_ptr.push_back(base + L.seg_ptr[k]);
             for (int c : L.idx) A.idx.push_back(c + off);
             A.off.insert(A.off.end(), L.off.begin(), L.off.end());
+            A.prev.insert(A.prev.end(), L.prev.begin(), L.prev.end());
             A.coef.insert(A.coef.end(), L.coef.begin(), L.coef.end());
         }
     }
@@ -322,14 +323,19 @@ void launch_pair(EnsContext &C, int par, int row, hipStream_t s, hipEvent_t e0 =
         launch_ens_mixed(s, sh, C.in32(par), p, C.out32(q), C.ddom, L.first, L.second, tide);
     }
     if (e1) NEMO_HIP_CHECK(hipEventRecord(e1, s));
+    const int prev = C.paired ? par : q;  // the previous step's velocities
     if (C.prec == 64)
         launch_flather(s, C.fl_u.view(), C.fl_v.view(), static_cast<double *>(C.prog[q][2]),
                        static_cast<double *>(C.prog[q][3]),
+                       static_cast<const double *>(C.prog[prev][2]),
+                       static_cast<const double *>(C.prog[prev][3]),
                        static_cast<const double *>(C.prog[par][4]),
                        static_cast<const double *>(C.prog[par][5]));
     else
         launch_flather_f32(s, C.fl_u.view(), C.fl_v.view(), static_cast<float *>(C.prog[q][2]),
                            static_cast<float *>(C.prog[q][3]),
+                           static_cast<const float *>(C.prog[prev][2]),
+                           static_cast<const float *>(C.prog[prev][3]),
                            static_cast<const float *>(C.prog[par][4]),
                            static_cast<const float *>(C.prog[par][5]));
 }

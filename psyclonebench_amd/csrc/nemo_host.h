@@ -44,7 +44,7 @@ inline T *dev_alloc(size_t n)
 }
 
 struct HostFlather {
-    std::vector<int> seg_ptr{0}, idx, off;
+    std::vector<int> seg_ptr{0}, idx, off, prev;
     std::vector<double> coef;
 };
 
@@ -64,6 +64,15 @@ inline HostFlather build_flather(const int *tm, const double *h, const Geometry 
     const int y1 = along_i ? G.ye + 1 : G.ye;
     const int outer0 = along_i ? y0 : x0, outer1 = along_i ? y1 : x1;
     const int inner0 = along_i ? x0 : y0, inner1 = along_i ? x1 : y1;
+    // a face of the sweep's region that the sweep writes (and only it:
+    // such a face has one open and one wet cell, so neither momentum nor
+    // bc_solid writes it)
+    auto is_flather = [&](int ji, int jj) {
+        if (ji < x0 || ji > x1 || jj < y0 || jj > y1) return false;
+        const long c = (long)jj * W + ji;
+        const int mc = tm[c], mp = tm[c + stride];
+        return mc + mp > -1 && (mc < 0 || mp < 0);
+    };
     for (int o = outer0; o <= outer1; o++) {
         const size_t before = L.idx.size();
         for (int in = inner0; in <= inner1; in++) {
@@ -81,6 +90,8 @@ inline HostFlather build_flather(const int *tm, const double *h, const Geometry 
             const long q = c + d;
             L.idx.push_back((int)c);
             L.off.push_back(q >= 0 && q < G.total ? d : 0);
+            L.prev.push_back(d > 0 && (along_i ? is_flather(ji + 1, jj)
+                                               : is_flather(ji, jj + 1)));
             L.coef.push_back(std::sqrt(g / h[c]));
         }
         if (L.idx.size() > before) L.seg_ptr.push_back((int)L.idx.size());
@@ -89,7 +100,7 @@ inline HostFlather build_flather(const int *tm, const double *h, const Geometry 
 }
 
 struct DeviceFlather {
-    int *seg_ptr = nullptr, *idx = nullptr, *off = nullptr;
+    int *seg_ptr = nullptr, *idx = nullptr, *off = nullptr, *prev = nullptr;
     double *coef = nullptr;
     int nseg = 0;
 
@@ -101,6 +112,7 @@ struct DeviceFlather {
         seg_ptr = dev_alloc<int>(L.seg_ptr.size());
         idx = dev_alloc<int>(L.idx.size());
         off = dev_alloc<int>(L.off.size());
+        prev = dev_alloc<int>(L.prev.size());
         coef = dev_alloc<double>(L.coef.size());
         NEMO_HIP_CHECK(hipMemcpyAsync(seg_ptr, L.seg_ptr.data(),
                                       L.seg_ptr.size() * sizeof(int),
@@ -108,6 +120,8 @@ struct DeviceFlather {
         NEMO_HIP_CHECK(hipMemcpyAsync(idx, L.idx.data(), L.idx.size() * sizeof(int),
                                       hipMemcpyHostToDevice, s));
         NEMO_HIP_CHECK(hipMemcpyAsync(off, L.off.data(), L.off.size() * sizeof(int),
+                                      hipMemcpyHostToDevice, s));
+        NEMO_HIP_CHECK(hipMemcpyAsync(prev, L.prev.data(), L.prev.size() * sizeof(int),
                                       hipMemcpyHostToDevice, s));
         NEMO_HIP_CHECK(hipMemcpyAsync(coef, L.coef.data(),
                                       L.coef.size() * sizeof(double),
@@ -119,12 +133,13 @@ struct DeviceFlather {
         if (seg_ptr) NEMO_HIP_CHECK(hipFree(seg_ptr));
         if (idx) NEMO_HIP_CHECK(hipFree(idx));
         if (off) NEMO_HIP_CHECK(hipFree(off));
+        if (prev) NEMO_HIP_CHECK(hipFree(prev));
         if (coef) NEMO_HIP_CHECK(hipFree(coef));
-        seg_ptr = idx = off = nullptr;
+        seg_ptr = idx = off = prev = nullptr;
         coef = nullptr;
         nseg = 0;
     }
-    FlatherList view() const { return FlatherList{seg_ptr, idx, off, coef, nseg}; }
+    FlatherList view() const { return FlatherList{seg_ptr, idx, off, prev, coef, nseg}; }
 };
 
 inline void check_geometry(const Geometry &G)

@@ -462,9 +462,9 @@ __global__ __launch_bounds__(64) void k_fused_wave_a(
                           tide_tab ? tide_tab[tide_k] : tide_val);
 }
 
-// The aligned march with the loads issued two rows ahead (tile_f64's
-// PF = 1); MINW wavefronts per SIMD requested from the register allocator.
-template <int R, int OW, int MINW>
+// The aligned march with the loads issued PF+1 rows ahead (tile_f64's PF);
+// MINW wavefronts per SIMD requested from the register allocator.
+template <int R, int OW, int MINW, int PF = 1>
 __global__ __launch_bounds__(64, MINW) void k_fused_wave_p(
     StepIn<double> f, StepScalars<double> p, StepOut<double> o, int W, Bounds b,
     int nstrip, const double *__restrict__ tide_tab, int tide_k, double tide_val)
@@ -472,19 +472,73 @@ __global__ __launch_bounds__(64, MINW) void k_fused_wave_p(
     static_assert(OW % 8 == 0, "64-byte-aligned strips");
     const int logical = wave::xcd_tile((int)blockIdx.x, (int)gridDim.x);
     const int band = logical / nstrip, strip = logical - band * nstrip;
-    wave::tile_f64<R, OW, 1>(f, p, o, W, 0u, b, strip, band,
-                             tide_tab ? tide_tab[tide_k] : tide_val);
+    wave::tile_f64<R, OW, PF>(f, p, o, W, 0u, b, strip, band,
+                              tide_tab ? tide_tab[tide_k] : tide_val);
 }
 
-template <int R, int OW, int MINW>
+template <int R, int OW, int MINW, int PF = 1>
 void launch_wave_prefetch_t(hipStream_t s, const StepIn<double> &f,
                             const StepScalars<double> &p, const StepOut<double> &o, int W,
                             Bounds b, const double *tide_tab, int tide_k, double tide_val)
 {
     const int ncol = b.xe - b.xs + 3, nrow = b.ye - b.ys + 3;
     const int nstrip = (ncol + OW - 1) / OW, nband = (nrow + R - 1) / R;
-    k_fused_wave_p<R, OW, MINW><<<dim3((unsigned)(nstrip * nband)), dim3(64), 0, s>>>(
+    k_fused_wave_p<R, OW, MINW, PF><<<dim3((unsigned)(nstrip * nband)), dim3(64), 0, s>>>(
         f, p, o, W, b, nstrip, tide_tab, tide_k, tide_val);
+    NEMO_LAUNCH_CHECK();
+}
+
+// Persistent chunked march: one wavefront per (strip, chunk), a chunk being
+// `rows` consecutive rows of the strip. The host sizes the launch to the
+// wavefronts the GPU holds at once (the kernel's occupancy times the CU
+// count), so every wavefront marches one long run with its window kept (a
+// halo of 2 rows per run instead of per short band) and all of them finish
+// together. PF and U as in tile_f64_rows.
+template <int OW, int PF, int U>
+__global__ __launch_bounds__(64, 2) void k_fused_wave_c(
+    StepIn<double> f, StepScalars<double> p, StepOut<double> o, int W, Bounds b,
+    int nstrip, int rows, const double *__restrict__ tide_tab, int tide_k, double tide_val)
+{
+    static_assert(OW % 8 == 0, "64-byte-aligned strips");
+    const int logical = wave::xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const int chunk = logical / nstrip, strip = logical - chunk * nstrip;
+    const int jlo = b.ys - 1 + chunk * rows;
+    const int jhi = min(jlo + rows, b.ye + 2);
+    wave::tile_f64_rows<1 << 24, OW, PF, U>(f, p, o, W, 0u, b, strip, jlo, jhi,
+                                            tide_tab ? tide_tab[tide_k] : tide_val);
+}
+
+// Wavefronts of k_fused_wave_c<OW, PF, U> the current device holds at once.
+template <int OW, int PF, int U>
+int chunk_residency()
+{
+    constexpr int kMaxDevices = 64;
+    static int cache[kMaxDevices] = {};
+    int dev = 0;
+    NEMO_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) NEMO_FATAL("device index %d beyond %d", dev, kMaxDevices);
+    if (!cache[dev]) {
+        int per_cu = 0, ncu = 0;
+        NEMO_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, k_fused_wave_c<OW, PF, U>, 64, 0));
+        NEMO_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        cache[dev] = per_cu * ncu > 0 ? per_cu * ncu : 1;
+    }
+    return cache[dev];
+}
+
+template <int OW, int PF, int U>
+void launch_wave_chunk_t(hipStream_t s, const StepIn<double> &f,
+                         const StepScalars<double> &p, const StepOut<double> &o, int W,
+                         Bounds b, const double *tide_tab, int tide_k, double tide_val)
+{
+    const int ncol = b.xe - b.xs + 3, nrow = b.ye - b.ys + 3;
+    const int nstrip = (ncol + OW - 1) / OW;
+    const int want = max(1, chunk_residency<OW, PF, U>() / nstrip);
+    const int rows = (nrow + want - 1) / want;
+    const int nchunk = (nrow + rows - 1) / rows;
+    k_fused_wave_c<OW, PF, U><<<dim3((unsigned)(nstrip * nchunk)), dim3(64), 0, s>>>(
+        f, p, o, W, b, nstrip, rows, tide_tab, tide_k, tide_val);
     NEMO_LAUNCH_CHECK();
 }
 
@@ -518,7 +572,10 @@ void launch_wave_t(hipStream_t s, const StepIn<double> &f,
 // SIMD (the compiler keeps its registers within that budget), 5 / 6 the
 // overlapped-wavefront march with 56 / 48 output columns (64-byte-aligned
 // stores; with 48 the loads are aligned too), 7 / 8 the 56-column march with
-// its loads issued two rows ahead, compiled for >= 2 / 3 wavefronts per SIMD
+// its loads issued two rows ahead, compiled for >= 2 / 3 wavefronts per SIMD,
+// 9 / 10 the persistent chunked 56-column march with / without the loads two
+// rows ahead (rows 0: the chunk height is set at launch from the occupancy),
+// 11 the 56-column march with its loads issued three rows ahead
 struct MarchShape {
     int mx, rows, prefetch;
 };
@@ -547,7 +604,8 @@ static const MarchShape kShapes[] = {
     {64, 64, 4},  {64, 16, 5},  {64, 32, 5},  {64, 8, 5},   {64, 16, 6},
     {64, 32, 6},  {64, 4, 5},   {64, 8, 6},   {64, 4, 6},   {64, 2, 5},
     {64, 6, 5},   {64, 4, 7},   {64, 6, 7},   {64, 8, 7},   {64, 12, 7},
-    {64, 4, 8},   {64, 8, 8},
+    {64, 4, 8},   {64, 8, 8},   {64, 32, 7},  {64, 64, 7},  {64, 0, 9},
+    {64, 0, 9},   {64, 0, 10},  {64, 4, 11},  {64, 6, 11},  {64, 8, 11},
 };
 
 // Kernel names as the profiler reports them (template arguments included).
@@ -567,10 +625,14 @@ static const char *const kNames[] = {
     "k_fused_wave_a<16, 48>",        "k_fused_wave_a<32, 48>",
     "k_fused_wave_a<4, 56>",         "k_fused_wave_a<8, 48>",
     "k_fused_wave_a<4, 48>",         "k_fused_wave_a<2, 56>",
-    "k_fused_wave_a<6, 56>",         "k_fused_wave_p<4, 56, 2>",
-    "k_fused_wave_p<6, 56, 2>",      "k_fused_wave_p<8, 56, 2>",
-    "k_fused_wave_p<12, 56, 2>",     "k_fused_wave_p<4, 56, 3>",
-    "k_fused_wave_p<8, 56, 3>",
+    "k_fused_wave_a<6, 56>",         "k_fused_wave_p<4, 56, 2, 1>",
+    "k_fused_wave_p<6, 56, 2, 1>",      "k_fused_wave_p<8, 56, 2, 1>",
+    "k_fused_wave_p<12, 56, 2, 1>",     "k_fused_wave_p<4, 56, 3, 1>",
+    "k_fused_wave_p<8, 56, 3, 1>",      "k_fused_wave_p<32, 56, 2, 1>",
+    "k_fused_wave_p<64, 56, 2, 1>",     "k_fused_wave_c<56, 1, 8>",
+    "k_fused_wave_c<56, 1, 4>",      "k_fused_wave_c<56, 0, 4>",
+    "k_fused_wave_p<4, 56, 2, 2>",   "k_fused_wave_p<6, 56, 2, 2>",
+    "k_fused_wave_p<8, 56, 2, 2>",
 };
 static_assert(sizeof(kNames) / sizeof(kNames[0]) == sizeof(kShapes) / sizeof(kShapes[0]),
               "one kernel name per shape");
@@ -671,6 +733,30 @@ void launch_fused_march(hipStream_t s, int variant, const StepIn<double> &f,
         break;
     case 36:
         launch_wave_prefetch_t<8, 56, 3>(s, f, p, o, W, b, tide_tab, tide_k, tide_val);
+        break;
+    case 37:
+        launch_wave_prefetch_t<32, 56, 2>(s, f, p, o, W, b, tide_tab, tide_k, tide_val);
+        break;
+    case 38:
+        launch_wave_prefetch_t<64, 56, 2>(s, f, p, o, W, b, tide_tab, tide_k, tide_val);
+        break;
+    case 39:
+        launch_wave_chunk_t<56, 1, 8>(s, f, p, o, W, b, tide_tab, tide_k, tide_val);
+        break;
+    case 40:
+        launch_wave_chunk_t<56, 1, 4>(s, f, p, o, W, b, tide_tab, tide_k, tide_val);
+        break;
+    case 41:
+        launch_wave_chunk_t<56, 0, 4>(s, f, p, o, W, b, tide_tab, tide_k, tide_val);
+        break;
+    case 42:
+        launch_wave_prefetch_t<4, 56, 2, 2>(s, f, p, o, W, b, tide_tab, tide_k, tide_val);
+        break;
+    case 43:
+        launch_wave_prefetch_t<6, 56, 2, 2>(s, f, p, o, W, b, tide_tab, tide_k, tide_val);
+        break;
+    case 44:
+        launch_wave_prefetch_t<8, 56, 2, 2>(s, f, p, o, W, b, tide_tab, tide_k, tide_val);
         break;
     default: NEMO_FATAL("unknown fused march variant %d", variant);
     }

@@ -42,8 +42,10 @@ struct MixedOut {
 void launch_mixed_step(hipStream_t s, int rows, int cols, const MixedIn &f,
                        const StepScalars<float> &p, const MixedOut &o, int W,
                        Bounds b, const double *tide_tab, int tide_k, double tide_val);
+// ua_prev / va_prev: the previous step's velocities (see FlatherList)
 void launch_flather_f32(hipStream_t s, const FlatherList &lu, const FlatherList &lv,
-                        float *ua, float *va, const float *su, const float *sv);
+                        float *ua, float *va, const float *ua_prev, const float *va_prev,
+                        const float *su, const float *sv);
 void launch_to_f32(hipStream_t s, const double *src, float *dst, long n);
 void launch_split_f64(hipStream_t s, const double *src, float *hi, float *lo, long n);
 void launch_to_f64(hipStream_t s, const float *src, const float *lo, double *dst, long n);

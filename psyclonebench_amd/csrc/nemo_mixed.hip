@@ -49,26 +49,29 @@ __global__ __launch_bounds__(64) void k_mixed_wave_a(MixedIn f, StepScalars<floa
 
 // Flather over the same segment lists as the FP64 path, FP32 values.
 __global__ __launch_bounds__(64) void k_flather_f32(FlatherList lu, FlatherList lv,
-                                                    float *__restrict__ ua,
-                                                    float *__restrict__ va,
+                                                    float *ua, float *va,
+                                                    const float *ua_prev,
+                                                    const float *va_prev,
                                                     const float *__restrict__ su,
                                                     const float *__restrict__ sv)
 {
     int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     const FlatherList *L = &lu;
     float *vel = ua;
+    const float *vel_prev = ua_prev;
     const float *ssh = su;
     if (t >= lu.nseg) {
         t -= lu.nseg;
         if (t >= lv.nseg) return;
         L = &lv;
         vel = va;
+        vel_prev = va_prev;
         ssh = sv;
     }
     const int k1 = L->seg_ptr[t + 1];
     for (int k = L->seg_ptr[t]; k < k1; k++) {
         const int c = L->idx[k], off = L->off[k];
-        const float v_nb = off ? vel[c + off] : 0.0f;
+        const float v_nb = off ? (L->prev[k] ? vel_prev[c + off] : vel[c + off]) : 0.0f;
         const float s_nb = off ? ssh[c + off] : 0.0f;
         vel[c] = v_nb + (float)L->coef[k] * (ssh[c] - s_nb);
     }
@@ -143,12 +146,13 @@ void launch_mixed_step(hipStream_t s, int rows, int cols, const MixedIn &f,
 }
 
 void launch_flather_f32(hipStream_t s, const FlatherList &lu, const FlatherList &lv,
-                        float *ua, float *va, const float *su, const float *sv)
+                        float *ua, float *va, const float *ua_prev, const float *va_prev,
+                        const float *su, const float *sv)
 {
     const int n = lu.nseg + lv.nseg;
     if (n == 0) return;
-    k_flather_f32<<<dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s>>>(lu, lv, ua, va,
-                                                                      su, sv);
+    k_flather_f32<<<dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s>>>(
+        lu, lv, ua, va, ua_prev, va_prev, su, sv);
     NEMO_LAUNCH_CHECK();
 }
 

@@ -110,8 +110,9 @@ void launch_pair(MixedContext &C, int par, hipStream_t s, const double *tide_tab
     launch_mixed_step(s, C.rows, C.cols, C.inputs(par), C.scalars(), o, C.G.W,
                       Bounds{C.G.xs, C.G.xe, C.G.ys, C.G.ye}, tide_tab, k, tide_val);
     if (e1) NEMO_HIP_CHECK(hipEventRecord(e1, s));
-    launch_flather_f32(s, C.fl_u.view(), C.fl_v.view(), C.u[q], C.v[q], C.su[par],
-                       C.sv[par]);
+    const int prev = C.paired ? par : q;  // the previous step's velocities
+    launch_flather_f32(s, C.fl_u.view(), C.fl_v.view(), C.u[q], C.v[q], C.u[prev],
+                       C.v[prev], C.su[par], C.sv[par]);
 }
 
 void first_step(MixedContext &C, int istep, hipStream_t s)

@@ -34,8 +34,9 @@ double g_amp_tide = 0.2;
 double g_omega_tide = 2.0 * 3.14159 / (12.42 * 3600.0);
 int g_mode = 0;
 // -1: tile kernel; >= 0: row-march variant. The default is the fastest
-// measured at 4096^2: 4-row bands of 56-column, 64-byte-aligned strips.
-int g_fused_variant = 26;
+// measured at 4096^2: 4-row bands of 56-column, 64-byte-aligned strips with
+// the loads issued two rows ahead.
+int g_fused_variant = 31;
 bool g_kernel_timing = false;
 double g_fused_ms = -1.0;
 
@@ -209,8 +210,8 @@ void step_unfused(Context &C, int istep, hipStream_t s)
                     Region{G.xs - 1, G.xe, G.ys - 1, G.ye + 1});
     launch_bc_solid(s, C.mask(), C.d(A_VA), W, W,
                     Region{G.xs - 1, G.xe + 1, G.ys - 1, G.ye});
-    launch_flather(s, C.fl_u.view(), C.fl_v.view(), C.d(A_UA), C.d(A_VA),
-                   C.d(A_SSHN_U), C.d(A_SSHN_V));
+    launch_flather(s, C.fl_u.view(), C.fl_v.view(), C.d(A_UA), C.d(A_VA), C.d(A_UA),
+                   C.d(A_VA), C.d(A_SSHN_U), C.d(A_SSHN_V));
     launch_copy3(s, C.d(A_UN), C.d(A_UA), C.d(A_VN), C.d(A_VA), C.d(A_SSHN_T),
                  C.d(A_SSHA_T), W, Region{G.xs - 1, G.xe + 1, G.ys - 1, G.ye});
     launch_next_ssh(s, C.mask(), C.d(A_SSHN_T), C.d(A_AREA_T), C.d(A_AREA_U),
@@ -235,8 +236,11 @@ void launch_fused_pair(Context &C, int par, hipStream_t s, const double *tide_ta
         launch_fused_march(s, g_fused_variant, C.inputs(par), C.scalars(), o,
                            C.G.W, bd, tide_tab, tide_k, tide_val);
     if (e1) NEMO_HIP_CHECK(hipEventRecord(e1, s));
-    launch_flather(s, C.fl_u.view(), C.fl_v.view(), C.U(q), C.V(q), C.SU(par),
-                   C.SV(par));
+    // the previous step's velocities: set `par` once the sets are paired; on
+    // the first step the after arrays themselves, as the reference has them
+    const int prev = C.paired ? par : q;
+    launch_flather(s, C.fl_u.view(), C.fl_v.view(), C.U(q), C.V(q), C.U(prev), C.V(prev),
+                   C.SU(par), C.SV(par));
 }
 
 void step_fused(Context &C, int istep, hipStream_t s)

@@ -32,11 +32,16 @@ struct StepOut {
 // row for U faces, one per column for V faces (bc_flather_u/v read the
 // neighbour face the serial sweep may already have updated). Entry k of a
 // segment updates vel[idx[k]] from vel[idx[k] + off[k]]; off == 0 marks a
-// neighbour outside the allocation (read as 0).
+// neighbour outside the allocation (read as 0). prev[k] != 0 marks a
+// neighbour that is itself a Flather face later in the sweep: the serial
+// sweep reads it before updating it, so it reads the value that face held
+// at the end of the previous step, which a caller that ping-pongs its
+// buffers keeps in the previous step's set (vel_prev), not in `vel`.
 struct FlatherList {
     const int *seg_ptr;   // nseg + 1
     const int *idx;       // face index
     const int *off;       // +-1 (U) or +-W (V), or 0
+    const int *prev;      // 1: read the neighbour from vel_prev
     const double *coef;   // sqrt(g / h) at the face
     int nseg;
 };
@@ -54,8 +59,12 @@ void launch_bc_ssh(hipStream_t s, const int *tmask, double *ssha, int W,
                    Region r, double tide);
 void launch_bc_solid(hipStream_t s, const int *tmask, double *vel, int W,
                      int stride, Region r);
+// ua_prev / va_prev: the velocities at the end of the previous step (the
+// previous ping-pong set, or ua / va themselves when they are updated in
+// place or on a first step that writes into the after arrays).
 void launch_flather(hipStream_t s, const FlatherList &lu, const FlatherList &lv,
-                    double *ua, double *va, const double *sshn_u,
+                    double *ua, double *va, const double *ua_prev,
+                    const double *va_prev, const double *sshn_u,
                     const double *sshn_v);
 void launch_copy3(hipStream_t s, double *un, const double *ua, double *vn,
                   const double *va, double *sshn_t, const double *ssha_t,

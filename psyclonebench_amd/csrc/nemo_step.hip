@@ -133,26 +133,29 @@ __global__ __launch_bounds__(BX *BY) void k_next_ssh(
 // column (V); different segments touch disjoint faces.
 // boundary_conditions_kern.c:282-292 (U), :322-332 (V).
 __global__ __launch_bounds__(64) void k_flather(FlatherList lu, FlatherList lv,
-                                                double *__restrict__ ua,
-                                                double *__restrict__ va,
+                                                double *ua, double *va,
+                                                const double *ua_prev,
+                                                const double *va_prev,
                                                 const double *__restrict__ su,
                                                 const double *__restrict__ sv)
 {
     int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     const FlatherList *L = &lu;
     double *vel = ua;
+    const double *vel_prev = ua_prev;
     const double *ssh = su;
     if (t >= lu.nseg) {
         t -= lu.nseg;
         if (t >= lv.nseg) return;
         L = &lv;
         vel = va;
+        vel_prev = va_prev;
         ssh = sv;
     }
     const int k1 = L->seg_ptr[t + 1];
     for (int k = L->seg_ptr[t]; k < k1; k++) {
         const int c = L->idx[k], o = L->off[k];
-        const double v_nb = o ? vel[c + o] : 0.0;
+        const double v_nb = o ? (L->prev[k] ? vel_prev[c + o] : vel[c + o]) : 0.0;
         const double s_nb = o ? ssh[c + o] : 0.0;
         vel[c] = v_nb + L->coef[k] * (ssh[c] - s_nb);
     }
@@ -313,13 +316,14 @@ void launch_bc_solid(hipStream_t s, const int *tmask, double *vel, int W,
 }
 
 void launch_flather(hipStream_t s, const FlatherList &lu, const FlatherList &lv,
-                    double *ua, double *va, const double *sshn_u,
+                    double *ua, double *va, const double *ua_prev,
+                    const double *va_prev, const double *sshn_u,
                     const double *sshn_v)
 {
     const int n = lu.nseg + lv.nseg;
     if (n == 0) return;
     k_flather<<<dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s>>>(
-        lu, lv, ua, va, sshn_u, sshn_v);
+        lu, lv, ua, va, ua_prev, va_prev, sshn_u, sshn_v);
     NEMO_LAUNCH_CHECK();
 }
 

@@ -99,32 +99,34 @@ __device__ __forceinline__ int xcd_tile(int bid, int nblk)
 //
 // PF = 0: the loads of row j+1 are issued at the top of row j, which needs
 // them, so a wavefront waits for them and only other wavefronts overlap
-// that wait. PF = 1: the window holds one more row and the loads of row j+2
-// (and the row-j+1 values of the per-point arrays) are issued at the top of
-// row j, two rows before use, behind a scheduling barrier that keeps the
-// compiler from moving them next to their use; the band loop is then fully
-// unrolled so that the window shifts are register renaming, never copies of
-// values still in flight.
-template <int R, int OW = OUT, int PF = 0>
-__device__ __forceinline__ void tile_f64(const StepIn<double> &f,
-                                         const StepScalars<double> &p,
-                                         const StepOut<double> &o, int W,
-                                         unsigned base,
-                                         const Bounds &b, int strip, int band,
-                                         double tide)
+// that wait. PF = k > 0: the window holds k more rows and the loads of row
+// j+1+k (and the row-j+k values of the per-point arrays) are issued at the
+// top of row j, k rows of work before use, behind a scheduling barrier that
+// keeps the compiler from moving them next to their use; the band loop is then fully
+// unrolled (bands up to 16 rows) so that the window shifts are register
+// renaming, never copies of values still in flight. Longer runs of rows are
+// unrolled by U; the shifts at the loop's back edge then wait once per U
+// rows for the loads in flight.
+//
+// tile_f64_rows walks rows jlo .. jhi-1 (at most R of them) of one strip;
+// tile_f64 is the band form, rows ys-1 + band*R onwards.
+template <int R, int OW = OUT, int PF = 0,
+          int U = (PF >= 1 ? (R <= 16 ? R : 8) : (R <= 8 ? R : 1))>
+__device__ __forceinline__ void tile_f64_rows(const StepIn<double> &f,
+                                              const StepScalars<double> &p,
+                                              const StepOut<double> &o, int W,
+                                              unsigned base, const Bounds &b, int strip,
+                                              int jlo, int jhi, double tide)
 {
     constexpr int L0 = (64 - OW) / 2;
     static_assert(L0 >= 1 && L0 + OW + 2 <= 64, "one neighbour lane below, two above");
-    static_assert(PF == 0 || PF == 1, "prefetch depth 0 or 1");
-    static_assert(PF == 0 || R <= 16, "the prefetching band loop is fully unrolled");
+    static_assert(PF >= 0 && PF <= 2, "prefetch depth 0, 1 or 2");
     using T = double;
     const int xmax = b.xe + 1, ymax = b.ye + 1;
     const int lane = (int)(threadIdx.x & 63);  // lane within the wavefront
     const int i = b.xs - 1 + strip * OW + lane - L0;  // may be < 0 or > xmax
     const bool owner = lane >= L0 && lane < L0 + OW && i <= xmax;
     const int ci = min(max(i, 0), xmax);                 // clamped load column
-    const int jlo = b.ys - 1 + band * R;
-    const int jhi = min(jlo + R, ymax + 1);
     auto rowoff = [&](int j) { return base + (unsigned)(min(max(j, 0), ymax) * W); };
 
     // own-column window: 3-slot rows j-1, j, j+1; 2-slot rows j, j+1;
@@ -199,27 +201,28 @@ __device__ __forceinline__ void tile_f64(const StepIn<double> &f,
     tm[0] = ld32(f.tmask, rowoff(jlo - 1) + ci);
     tm[1] = ld32(f.tmask, rowoff(jlo) + ci);
     tm[2] = ld32(f.tmask, rowoff(jlo + 1) + ci);
-    if constexpr (PF == 1) {
-        load3(jlo + 1, 2);
-        load2(jlo + 1, 1);
-        tm[3] = ld32(f.tmask, rowoff(jlo + 2) + ci);
-        load1(jlo, 0);
+    // with PF = k, rows up to jlo+k are in flight before the first row
+#pragma unroll
+    for (int k = 1; k <= PF; k++) {
+        load3(jlo + k, 1 + k);
+        load2(jlo + k, k);
+        tm[2 + k] = ld32(f.tmask, rowoff(jlo + 1 + k) + ci);
+        load1(jlo + k - 1, k - 1);
     }
     T ssh_c = new_ssh(1, 0, 1);  // T(i, jlo)
 
-    // short bands: the row loop is unrolled, so the window shifts at its
-    // end become register renaming instead of moves
-    constexpr int UNROLL = (PF == 1 || R <= 8) ? R : 1;
-#pragma unroll UNROLL
+    // the row loop is unrolled by U, so the window shifts inside an unrolled
+    // group become register renaming instead of moves
+#pragma unroll U
     for (int jj = 0; jj < R; jj++) {
         const int j = jlo + jj;
         if (j >= jhi) break;  // wavefront-uniform
-        if constexpr (PF == 1) {
-            if (jj + 1 < R) {  // rows the band still needs, two ahead
-                load3(j + 2, 3);
-                load2(j + 2, 2);
-                tm[4] = ld32(f.tmask, rowoff(j + 3) + ci);
-                load1(j + 1, 1);
+        if constexpr (PF >= 1) {
+            if (jj + PF < R && j + PF < jhi) {  // rows the run still needs, PF+1 ahead
+                load3(j + 1 + PF, 2 + PF);
+                load2(j + 1 + PF, 1 + PF);
+                tm[3 + PF] = ld32(f.tmask, rowoff(j + 2 + PF) + ci);
+                load1(j + PF, PF);
             }
             __builtin_amdgcn_sched_barrier(0);
         } else {
@@ -388,6 +391,19 @@ __device__ __forceinline__ void tile_f64(const StepIn<double> &f,
 #undef NEMO_SHIFT
         ssh_c = ssh_n;
     }
+}
+
+template <int R, int OW = OUT, int PF = 0>
+__device__ __forceinline__ void tile_f64(const StepIn<double> &f,
+                                         const StepScalars<double> &p,
+                                         const StepOut<double> &o, int W,
+                                         unsigned base,
+                                         const Bounds &b, int strip, int band,
+                                         double tide)
+{
+    const int jlo = b.ys - 1 + band * R;
+    const int jhi = min(jlo + R, b.ye + 2);
+    tile_f64_rows<R, OW, PF>(f, p, o, W, base, b, strip, jlo, jhi, tide);
 }
 
 struct Ssh {

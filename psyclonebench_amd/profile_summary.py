@@ -49,6 +49,9 @@ def main() -> None:
                    help="LABEL=KERNEL_SUBSTRING=ALG_BYTES_PER_LAUNCH (repeatable)")
     p.add_argument("--copy-bytes", type=float,
                    help="bytes k_copy3 reads (= writes) per launch")
+    p.add_argument("--counters", action="append", default=[],
+                   help="a PMC pass's result database (repeatable): mean value per "
+                        "dispatch of each of its counters, per kernel")
     p.add_argument("--flops", action="append", default=[],
                    help="LABEL=KERNEL_SUBSTRING=FLOP_PER_LAUNCH (repeatable): achieved "
                         "TFLOP/s from the trace's average duration")
@@ -71,6 +74,19 @@ def main() -> None:
         print(f"[{label}] flop_per_launch: {float(flop):.4g}")
         print(f"[{label}] avg_duration_us (kernel trace): {dur_us:.2f}")
         print(f"[{label}] achieved_TFLOPs: {float(flop) / (dur_us * 1e-6) / 1e12:.1f}")
+
+    for db in a.counters:
+        c = sqlite3.connect(db)
+        rows = c.execute("select kernel_name, counter_name, count(*), avg(value) from "
+                         "counters_collection group by kernel_name, counter_name").fetchall()
+        names = sorted({r[1] for r in rows})
+        kernels = sorted({r[0] for r in rows})
+        val = {(r[0], r[1]): float(r[3]) for r in rows}
+        print()
+        print(f"## PMC counters, mean per dispatch ({', '.join(names)})")
+        for k in kernels:
+            cells = "  ".join(f"{n}={val.get((k, n), float('nan')):.4g}" for n in names)
+            print(f"{_short(k)[:40]:<40} {cells}")
 
     if not a.report:
         return

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import assert_bitwise, rel_err
+from helpers import assert_bitwise, live_state_model, rel_err
 from psyclonebench_amd import _native
 from psyclonebench_amd.ensemble import EnsembleStepper, ensemble_configs
 from psyclonebench_amd.model import Model
@@ -188,4 +188,42 @@ def test_ensemble_256_domains_sample_equals_single_domain_fused():
             assert_bitwise(ens.fetch(d, n)[:g.N - 1], st.fetch(n)[:g.N - 1],
                            f"domain {d} {n}")
         st.release()
+    ens.release()
+
+
+def test_ensemble_random_masks_match_oracle_and_mixed():
+    """Three domains with every array random and random {-1, 0, 1} masks
+    (wet cells between open cells: bc_flather reads a face's value from the
+    end of the previous step), 8 steps. FP64: each domain bitwise the
+    oracle on all fields. FP32: each domain bitwise the single-domain mixed
+    form."""
+    models = [live_state_model(n, seed=60 + n, random_mask=True) for n in (40, 77, 96)]
+    nsteps = 8
+    ens = EnsembleStepper([m.copy() for m in models], precision=64)
+    ens.run(nsteps)
+    for d, m in enumerate(models):
+        g = m.grid
+        host = [a.copy() for a in m.arrays()]
+        oracle.set_tide(m.cfg.tide_amp, m.cfg.tide_omega)
+        try:
+            oracle.run_steps(host, 1, nsteps, g.bounds, g.W, m.scalars)
+        finally:
+            oracle.oracle_lib().oracle_reset_tide()
+        ref = dict(zip(oracle.ARRAY_NAMES, host))
+        for n in ("sshn_u", "sshn_v"):
+            assert_bitwise(ens.fetch(d, n), ref[n], f"f64 domain {d} {n}")
+        for n in ("sshn_t", "un", "vn"):
+            assert_bitwise(ens.fetch(d, n)[:g.N - 1, :g.M], ref[n][:g.N - 1, :g.M],
+                           f"f64 domain {d} {n}")
+    ens.release()
+    ens = EnsembleStepper([m.copy() for m in models], precision=32)
+    ens.run(nsteps)
+    for d, m in enumerate(models):
+        g = m.grid
+        single = MixedStepper(m.copy())
+        single.run(nsteps)
+        for n in FIELDS:
+            assert_bitwise(ens.fetch(d, n)[:g.N - 1, :g.M], single.fetch(n)[:g.N - 1, :g.M],
+                           f"f32 domain {d} {n}")
+        single.release()
     ens.release()

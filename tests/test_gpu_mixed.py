@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import assert_bitwise, rel_err
+from helpers import assert_bitwise, live_state_model, rel_err
 from psyclonebench_amd import _native
 from psyclonebench_amd.model import (MASK_CLOSED, Model, NemoConfig, c1_config, c2_config)
 from psyclonebench_amd.psy import MixedStepper, TimeStepper
@@ -144,3 +144,25 @@ def test_mixed_4096_error_report():
     f64.release()
     mp.release()
     assert report[1000]["sshn_t"][0] <= SSH_RELMAX_TOL, report
+
+
+def test_mixed_random_mask_against_fp64():
+    """96^2, every array random and a random {-1, 0, 1} mask (wet cells
+    between open cells, where bc_flather reads a face's value from the end
+    of the previous step), 8 steps of the mixed form against the FP64 fused
+    form (bitwise the oracle), over the whole written region. Same bounds
+    as the physical cases: ssh 1e-5, velocities 1e-4 (relmax)."""
+    m = live_state_model(96, seed=52, random_mask=True)
+    g = m.grid
+    ref = TimeStepper(m.copy(), mode=2)
+    mp = MixedStepper(m.copy(), rows=4, cols=56)
+    ref.run(8)
+    mp.run(8)
+    sl = (slice(0, g.N - 1), slice(0, g.M))
+    for n, tol in (("sshn_t", SSH_RELMAX_TOL), ("un", VEL_RELMAX_TOL), ("vn", VEL_RELMAX_TOL),
+                   ("sshn_u", SSH_RELMAX_TOL), ("sshn_v", SSH_RELMAX_TOL)):
+        rmax, rrms = rel_err(mp.fetch(n)[sl], ref.fetch(n)[sl])
+        print(f"random mask, 8 steps: {n} relmax {rmax:.2e} relrms {rrms:.2e}", flush=True)
+        assert rmax <= tol, (n, rmax)
+    ref.release()
+    mp.release()

@@ -33,7 +33,7 @@ MODE_AUTO, MODE_UNFUSED, MODE_FUSED = 0, 1, 2
 @pytest.fixture(autouse=True)
 def _release_contexts():
     # every test starts on the variant the test before it found (the
-    # library default, 26, unless a test sets one) and leaves it that way
+    # library default, 31, unless a test sets one) and leaves it that way
     lib = _native.hip_lib()
     variant = lib.nemo_hip_fused_variant()
     yield
@@ -44,7 +44,7 @@ def _release_contexts():
 
 def _variants():
     # -1 = tile kernel, 0.. = row-march shapes (count fixed by the library)
-    return list(range(-1, 37))
+    return list(range(-1, 45))
 
 
 def test_gpu_visible():
@@ -314,6 +314,29 @@ def _assert_live_equal(out: dict, ref: dict, g, what: str) -> None:
         assert_bitwise(out[n], ref[n], f"{what} {n}")
     for n in NOW_OUT:
         assert_bitwise(out[n][:g.N - 1, :g.M], ref[n][:g.N - 1, :g.M], f"{what} {n}")
+
+
+@pytest.mark.parametrize("variant", _variants())
+def test_fused_random_mask_many_steps_every_variant(variant):
+    """A 96^2 grid with every array random and a random {-1, 0, 1} mask, 8
+    fused steps (graph replay) against the serial oracle, bitwise on all
+    eight outputs. A random mask has wet cells between two open cells, where
+    bc_flather reads a neighbour face the serial sweep has not yet updated:
+    the value that face held at the end of the previous step, which the
+    ping-pong buffers keep in the previous step's set."""
+    _native.hip_lib().nemo_hip_set_fused_variant(variant)
+    m = live_state_model(96, seed=51, random_mask=True)
+    g = m.grid
+    host = [a.copy() for a in m.arrays()]
+    oracle.set_tide(m.cfg.tide_amp, m.cfg.tide_omega)
+    try:
+        oracle.run_steps(host, 1, 8, g.bounds, g.W, m.scalars)
+    finally:
+        oracle.oracle_lib().oracle_reset_tide()
+    ref = dict(zip(oracle.ARRAY_NAMES, host))
+    out, mode = _gpu_run(m.copy(), 8, MODE_FUSED)
+    assert mode == MODE_FUSED
+    _assert_live_equal(out, ref, g, f"variant {variant}")
 
 
 @pytest.mark.slow
