@@ -112,6 +112,26 @@ NEMO_API void c_invoke_time_step_host(NEMO_HOST_ARRAYS, int istep, int internal_
                              double cbfr, double visc, double omega,
                              double d2r, double g);
 
+/* Allocation-free step. No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops). One step with the semantics of c_invoke_time_step_host
+ * (psykal_cpp/time_step.cpp:53-136, every sweep in place, now <- after over
+ * the reference's copy region), but the 25 array arguments are DEVICE
+ * pointers the caller owns, with total_size elements each (the reference
+ * layout), and the step is queued on `stream` (a hipStream_t; NULL = the
+ * null stream) without waiting for it. The library allocates nothing: the
+ * per-point Coriolis factors, evaluated on the device at every call, go to
+ * a static device scratch array of 32 MiB per GPU, so total_size is at most
+ * 2097152 (2^21) elements; larger grids end with a message and exit(1).
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (other streams, other threads) race. The tide is nemo_hip_set_tide's. */
+NEMO_API void c_invoke_time_step_noalloc(NEMO_HOST_ARRAYS, int istep,
+                                         int internal_xstart, int internal_xstop,
+                                         int internal_ystart, int internal_ystop,
+                                         int width, int total_size, double rdt,
+                                         double cbfr, double visc, double omega,
+                                         double d2r, double g, void *stream);
+
 /* ---- configuration, timing, teardown (no reference counterpart) ------- */
 
 /* Tidal forcing of bc_ssh: ssha = amp * sin(omega_tide * istep * rdt).

@@ -80,6 +80,8 @@ def hip_lib() -> C.CDLL:
         lib.c_invoke_time_step.argtypes = _HOST + [_vpp] * N_ARRAYS + _SCAL_DEV
         lib.c_invoke_time_steps.argtypes = [C.c_int] + _HOST + [_vpp] * N_ARRAYS + _SCAL_DEV
         lib.c_invoke_time_step_host.argtypes = _HOST + _SCAL_HOST
+        lib.c_invoke_time_step_noalloc.argtypes = [C.c_void_p] * N_ARRAYS + _SCAL_DEV + \
+            [C.c_void_p]
         lib.c_read_from_device.argtypes = [C.c_void_p, C.c_void_p] + [C.c_int] * 4 + [C.c_bool]
         lib.c_write_to_device.argtypes = [C.c_void_p, C.c_void_p] + [C.c_int] * 4 + [C.c_bool]
         lib.nemo_hip_set_tide.argtypes = [C.c_double, C.c_double]

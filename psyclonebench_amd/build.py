@@ -43,7 +43,7 @@ HIPCC_FLAGS = [
 
 HIP_SOURCES = ["nemo_step.hip", "nemo_march.hip", "nemo_psy.cpp", "nemo_mixed.hip",
                "nemo_mixed_api.cpp", "nemo_ensemble.hip", "nemo_ensemble_api.cpp",
-               "nemo_closure.hip", "nemo_closure_api.cpp"]
+               "nemo_closure.hip", "nemo_closure_api.cpp", "nemo_noalloc.hip"]
 
 
 def _hipcc() -> str:

@@ -204,6 +204,21 @@ def test_32_bit_offset_limit_is_rejected_before_any_device_call(form):
     assert "32-bit offsets" in r.stderr, r.stderr[-2000:]
 
 
+def test_noalloc_step_rejects_grids_beyond_its_scratch():
+    """c_invoke_time_step_noalloc keeps its per-point scratch in a fixed
+    static array of 2^21 elements per field: a larger total_size ends with
+    the library's message and exit(1) before any device call."""
+    import subprocess
+    import sys
+    code = _BIG_GRID_CALL.format(W=2048, ye=1100, call=(
+        "lib.c_invoke_time_step_noalloc(*([None] * 25), *tail, None)"))
+    root = Path(__file__).resolve().parents[1]
+    r = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 1, (r.returncode, r.stderr[-2000:])
+    assert "exceeds the 2097152 elements" in r.stderr, r.stderr[-2000:]
+
+
 @pytest.mark.parametrize("call", ["c_read_from_device(None, None, 1, 1, -3, 4, True)",
                                   "c_write_to_device(None, None, 1, 1, 8, 8, True)"])
 def test_copy_callbacks_reject_bad_arguments(call):
