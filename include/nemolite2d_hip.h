@@ -210,6 +210,10 @@ NEMO_API void nemo_mp_set_rows(void *handle, int rows);
  * 56 (every strip's stores start on a 32-byte boundary) or 48 (on a 64-byte
  * boundary). */
 NEMO_API void nemo_mp_set_columns(void *handle, int cols);
+/* 1: the kernel that issues its loads two rows ahead (rows per band 4 or 8
+ * only); 0 (default): the loads of a row issued where the row starts. Both
+ * compute the same bits. */
+NEMO_API void nemo_mp_set_prefetch(void *handle, int prefetch);
 /* Blocking read-back as FP64 (total_size values): field 0 sshn_t (hi+lo),
  * 1 un, 2 vn, 3 sshn_u, 4 sshn_v. */
 NEMO_API void nemo_mp_read(void *handle, int field, double *out);

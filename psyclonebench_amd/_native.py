@@ -103,6 +103,7 @@ def hip_lib() -> C.CDLL:
         lib.nemo_mp_kernel_ms.restype = C.c_double
         lib.nemo_mp_set_rows.argtypes = [C.c_void_p, C.c_int]
         lib.nemo_mp_set_columns.argtypes = [C.c_void_p, C.c_int]
+        lib.nemo_mp_set_prefetch.argtypes = [C.c_void_p, C.c_int]
         lib.nemo_mp_read.argtypes = [C.c_void_p, C.c_int, _dp]
         lib.nemo_mp_destroy.argtypes = [C.c_void_p]
         vp = C.c_void_p

@@ -39,7 +39,8 @@ struct MixedOut {
 // rows per band: 4, 8, 16, 32, 48, 64, 80, 96 or 128; output columns per
 // wavefront: 61 (overlapped strips), 56 (32-byte-aligned stores) or 48
 // (64-byte-aligned stores)
-void launch_mixed_step(hipStream_t s, int rows, int cols, const MixedIn &f,
+// prefetch 1: the kernel that issues its loads two rows ahead (rows 4 or 8)
+void launch_mixed_step(hipStream_t s, int rows, int cols, int prefetch, const MixedIn &f,
                        const StepScalars<float> &p, const MixedOut &o, int W,
                        Bounds b, const double *tide_tab, int tide_k, double tide_val);
 // ua_prev / va_prev: the previous step's velocities (see FlatherList)

@@ -47,6 +47,21 @@ __global__ __launch_bounds__(64) void k_mixed_wave_a(MixedIn f, StepScalars<floa
                             wave::split(tide_tab ? tide_tab[tide_k] : tide_val));
 }
 
+// The OW-column march with the loads issued two rows ahead (tile_mixed's
+// PF = 1; OW = 61, 56 or 48).
+template <int R, int OW>
+__global__ __launch_bounds__(64) void k_mixed_wave_p(MixedIn f, StepScalars<float> p,
+                                                     MixedOut o, int W, Bounds b,
+                                                     int nstrip,
+                                                     const double *__restrict__ tide_tab,
+                                                     int tide_k, double tide_val)
+{
+    const int logical = wave::xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const int band = logical / nstrip, strip = logical - band * nstrip;
+    wave::tile_mixed<R, OW, 1>(f, p, o, W, 0u, b, strip, band,
+                               wave::split(tide_tab ? tide_tab[tide_k] : tide_val));
+}
+
 // Flather over the same segment lists as the FP64 path, FP32 values.
 __global__ __launch_bounds__(64) void k_flather_f32(FlatherList lu, FlatherList lv,
                                                     float *ua, float *va,
@@ -107,7 +122,7 @@ inline dim3 blocks_for(long n) { return dim3((unsigned)((n + 255) / 256)); }
 
 }  // namespace
 
-void launch_mixed_step(hipStream_t s, int rows, int cols, const MixedIn &f,
+void launch_mixed_step(hipStream_t s, int rows, int cols, int prefetch, const MixedIn &f,
                        const StepScalars<float> &p, const MixedOut &o, int W,
                        Bounds b, const double *tide_tab, int tide_k, double tide_val)
 {
@@ -115,6 +130,30 @@ void launch_mixed_step(hipStream_t s, int rows, int cols, const MixedIn &f,
         NEMO_FATAL("mixed step: unsupported columns per wavefront %d", cols);
     const int ncol = b.xe - b.xs + 3, nrow = b.ye - b.ys + 3;
     const int nstrip = (ncol + cols - 1) / cols;
+    if (prefetch) {
+        switch (rows) {
+#define NEMO_MIXED_PF_CASE(RR)                                                       \
+    case RR: {                                                                       \
+        const dim3 grid((unsigned)(nstrip * ((nrow + RR - 1) / RR)));                \
+        if (cols == 48)                                                              \
+            k_mixed_wave_p<RR, 48><<<grid, dim3(64), 0, s>>>(                        \
+                f, p, o, W, b, nstrip, tide_tab, tide_k, tide_val);                  \
+        else if (cols == 56)                                                         \
+            k_mixed_wave_p<RR, 56><<<grid, dim3(64), 0, s>>>(                        \
+                f, p, o, W, b, nstrip, tide_tab, tide_k, tide_val);                  \
+        else                                                                         \
+            k_mixed_wave_p<RR, wave::OUT><<<grid, dim3(64), 0, s>>>(                 \
+                f, p, o, W, b, nstrip, tide_tab, tide_k, tide_val);                  \
+        break;                                                                       \
+    }
+            NEMO_MIXED_PF_CASE(4)
+            NEMO_MIXED_PF_CASE(8)
+#undef NEMO_MIXED_PF_CASE
+        default: NEMO_FATAL("mixed step: the prefetching kernel takes 4 or 8 rows, not %d", rows);
+        }
+        NEMO_LAUNCH_CHECK();
+        return;
+    }
     switch (rows) {
 #define NEMO_MIXED_CASE(RR)                                                          \
     case RR: {                                                                       \

@@ -22,6 +22,7 @@ struct MixedContext {
     double amp = 0.0, omega_tide = 0.0;
     int rows = 32;
     int cols = 61;                   // output columns per wavefront
+    int prefetch = 0;                // 1: loads issued two rows ahead (rows 4 or 8)
     float *cst[A_COUNT] = {};        // constant arrays (FP32); tmask in tm
     signed char *tm = nullptr;       // T mask, one byte a point
     float *fcor_u = nullptr, *fcor_v = nullptr;
@@ -31,7 +32,8 @@ struct MixedContext {
     bool paired = false;
     double *d_tide = nullptr, *h_tide = nullptr;
     int tide_cap = 0;
-    std::map<std::tuple<int, int, int, int>, hipGraphExec_t> graphs;  // n, parity, rows, cols
+    // n, parity, rows, cols, prefetch
+    std::map<std::tuple<int, int, int, int, int>, hipGraphExec_t> graphs;
     std::vector<hipEvent_t> ev;
 
     MixedIn inputs(int s) const
@@ -107,7 +109,7 @@ void launch_pair(MixedContext &C, int par, hipStream_t s, const double *tide_tab
     const int q = 1 - par;
     const MixedOut o{C.t_hi[q], C.t_lo[q], C.u[q], C.v[q], C.su[q], C.sv[q]};
     if (e0) NEMO_HIP_CHECK(hipEventRecord(e0, s));
-    launch_mixed_step(s, C.rows, C.cols, C.inputs(par), C.scalars(), o, C.G.W,
+    launch_mixed_step(s, C.rows, C.cols, C.prefetch, C.inputs(par), C.scalars(), o, C.G.W,
                       Bounds{C.G.xs, C.G.xe, C.G.ys, C.G.ye}, tide_tab, k, tide_val);
     if (e1) NEMO_HIP_CHECK(hipEventRecord(e1, s));
     const int prev = C.paired ? par : q;  // the previous step's velocities
@@ -219,7 +221,11 @@ void nemo_mp_set_rows(void *h, int rows)
         rows != 80 && rows != 96 && rows != 128)
         NEMO_FATAL("mixed form: rows per band must be 4, 8, 16, 32, 48, 64, 80, 96 or 128 "
                    "(got %d)", rows);
-    static_cast<MixedContext *>(h)->rows = rows;
+    MixedContext &C = *static_cast<MixedContext *>(h);
+    if (C.prefetch && rows != 4 && rows != 8)
+        NEMO_FATAL("mixed form: the prefetching kernel takes 4 or 8 rows per band (got %d)",
+                   rows);
+    C.rows = rows;
 }
 
 void nemo_mp_set_columns(void *h, int cols)
@@ -228,6 +234,17 @@ void nemo_mp_set_columns(void *h, int cols)
         NEMO_FATAL("mixed form: output columns per wavefront must be 61, 56 or 48 (got %d)",
                    cols);
     static_cast<MixedContext *>(h)->cols = cols;
+}
+
+void nemo_mp_set_prefetch(void *h, int prefetch)
+{
+    MixedContext &C = *static_cast<MixedContext *>(h);
+    if (prefetch != 0 && prefetch != 1)
+        NEMO_FATAL("mixed form: prefetch must be 0 or 1 (got %d)", prefetch);
+    if (prefetch && C.rows != 4 && C.rows != 8)
+        NEMO_FATAL("mixed form: the prefetching kernel takes 4 or 8 rows per band "
+                   "(rows are %d)", C.rows);
+    C.prefetch = prefetch;
 }
 
 static void mp_run(MixedContext &C, int nsteps, int istep, bool timed, double *ms)
@@ -258,7 +275,7 @@ static void mp_run(MixedContext &C, int nsteps, int istep, bool timed, double *m
         }
         *ms = sum / nsteps;
     } else {
-        auto key = std::make_tuple(nsteps, C.parity, C.rows, C.cols);
+        auto key = std::make_tuple(nsteps, C.parity, C.rows, C.cols, C.prefetch);
         auto it = C.graphs.find(key);
         if (it == C.graphs.end()) {
             hipGraph_t graph;

@@ -431,7 +431,9 @@ __device__ __forceinline__ float ssh_increment(const ContVals<float> &v, float r
 // compensated (hi, lo) pair accumulated in FP64.
 // Owned columns as in tile_f64: lanes L0 .. L0+OW-1, L0 = (64 - OW) / 2
 // (OW = 48 starts every strip's FP32 stores on a 64-byte boundary).
-template <int R, int OW = OUT>
+// PF as in tile_f64_rows: PF = k > 0 issues the loads of row j+1+k at the
+// top of row j, behind a scheduling barrier, in a fully unrolled band loop.
+template <int R, int OW = OUT, int PF = 0>
 __device__ __forceinline__ void tile_mixed(const MixedIn &f,
                                            const StepScalars<float> &p,
                                            const MixedOut &o, int W,
@@ -441,6 +443,8 @@ __device__ __forceinline__ void tile_mixed(const MixedIn &f,
 {
     constexpr int L0 = (64 - OW) / 2;
     static_assert(L0 >= 1 && L0 + OW + 2 <= 64, "one neighbour lane below, two above");
+    static_assert(PF >= 0 && PF <= 2, "prefetch depth 0, 1 or 2");
+    static_assert(PF == 0 || R <= 16, "the prefetching band loop is fully unrolled");
     using T = float;
     const int xmax = b.xe + 1, ymax = b.ye + 1;
     const int lane = (int)(threadIdx.x & 63);  // lane within the wavefront
@@ -451,10 +455,13 @@ __device__ __forceinline__ void tile_mixed(const MixedIn &f,
     const int jhi = min(jlo + R, ymax + 1);
     auto rowoff = [&](int j) { return base + (unsigned)(min(max(j, 0), ymax) * W); };
 
-    T un[3], vn[3], hu[3], su[3], hv[3], sv[3], e1v[3], e2u[3];
-    T ht[2], st[2], e1t[2], e2t[2], at[2];
-    T st_lo1;  // lo word of T(i, j+1)
-    int tm[4];
+    T un[3 + PF], vn[3 + PF], hu[3 + PF], su[3 + PF], hv[3 + PF], sv[3 + PF],
+        e1v[3 + PF], e2u[3 + PF];
+    T ht[2 + PF], st[2 + PF], e1t[2 + PF], e2t[2 + PF], at[2 + PF];
+    T lo[1 + PF];  // lo words of T(i, j+1) ..
+    int tm[4 + PF];
+    T dxu[1 + PF], dyv[1 + PF], fcu[1 + PF], fcv[1 + PF], aru[1 + PF], arv[1 + PF],
+        sau[1 + PF], sav[1 + PF];
 
     auto load3 = [&](int j, int s) {
         const unsigned c = rowoff(j) + ci;
@@ -498,12 +505,32 @@ __device__ __forceinline__ void tile_mixed(const MixedIn &f,
         return split((double)st[s2] + (double)lo + (double)inc);
     };
 
+    auto load1 = [&](int r, int s) {
+        const unsigned c = rowoff(r) + ci;
+        dxu[s] = ld32(f.dx_u, c);
+        dyv[s] = ld32(f.dy_v, c);
+        fcu[s] = ld32(f.fcor_u, c);
+        fcv[s] = ld32(f.fcor_v, c);
+        aru[s] = ld32(f.area_u, c);
+        arv[s] = ld32(f.area_v, c);
+        sau[s] = ld32(f.ssha_u, c);
+        sav[s] = ld32(f.ssha_v, c);
+    };
+
     load3(jlo - 1, 0);
     load3(jlo, 1);
     load2(jlo, 0);
     tm[0] = ld32(f.tmask, rowoff(jlo - 1) + ci);
     tm[1] = ld32(f.tmask, rowoff(jlo) + ci);
     tm[2] = ld32(f.tmask, rowoff(jlo + 1) + ci);
+#pragma unroll
+    for (int k = 1; k <= PF; k++) {
+        load3(jlo + k, 1 + k);
+        load2(jlo + k, k);
+        tm[2 + k] = ld32(f.tmask, rowoff(jlo + 1 + k) + ci);
+        lo[k - 1] = ld32(f.ssh_lo, rowoff(jlo + k) + ci);
+        load1(jlo + k - 1, k - 1);
+    }
     Ssh ssh_c = new_ssh(1, 0, 1, ld32(f.ssh_lo, rowoff(jlo) + ci));
     // neighbour values of window slots that the previous row already shifted
     // (slot k of row j is slot k+1 of row j-1): carried, not shifted again
@@ -514,20 +541,32 @@ __device__ __forceinline__ void tile_mixed(const MixedIn &f,
 
     // short bands: the row loop is unrolled, so the window shifts at its
     // end become register renaming instead of moves
-    constexpr int UNROLL = R <= 8 ? R : 1;
+    constexpr int UNROLL = (PF >= 1 || R <= 8) ? R : 1;
 #pragma unroll UNROLL
     for (int jj = 0; jj < R; jj++) {
         const int j = jlo + jj;
         if (j >= jhi) break;  // wavefront-uniform
-        load3(j + 1, 2);
-        load2(j + 1, 1);
-        st_lo1 = ld32(f.ssh_lo, rowoff(j + 1) + ci);
-        tm[3] = ld32(f.tmask, rowoff(j + 2) + ci);
+        if constexpr (PF >= 1) {
+            if (jj + PF < R && j + PF < jhi) {  // rows the band still needs
+                load3(j + 1 + PF, 2 + PF);
+                load2(j + 1 + PF, 1 + PF);
+                lo[PF] = ld32(f.ssh_lo, rowoff(j + 1 + PF) + ci);
+                tm[3 + PF] = ld32(f.tmask, rowoff(j + 2 + PF) + ci);
+                load1(j + PF, PF);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        } else {
+            load3(j + 1, 2);
+            load2(j + 1, 1);
+            lo[0] = ld32(f.ssh_lo, rowoff(j + 1) + ci);
+            tm[3] = ld32(f.tmask, rowoff(j + 2) + ci);
+            load1(j, 0);
+        }
         const unsigned c = rowoff(j) + ci;
-        const T dxu = ld32(f.dx_u, c), dyv = ld32(f.dy_v, c), fcu = ld32(f.fcor_u, c), fcv = ld32(f.fcor_v, c);
-        const T aru = ld32(f.area_u, c), arv = ld32(f.area_v, c), sau = ld32(f.ssha_u, c), sav = ld32(f.ssha_v, c);
+        const T dxu0 = dxu[0], dyv0 = dyv[0], fcu0 = fcu[0], fcv0 = fcv[0];
+        const T aru0 = aru[0], arv0 = arv[0], sau0 = sau[0], sav0 = sav[0];
 
-        const Ssh ssh_n = new_ssh(2, 1, 2, st_lo1);
+        const Ssh ssh_n = new_ssh(2, 1, 2, lo[0]);
         const float ssh_e = from_above(ssh_c.hi);
 
         if (owner && in_rect(i, j, b.xs, b.xe, b.ys, b.ye)) {
@@ -586,10 +625,10 @@ __device__ __forceinline__ void tile_mixed(const MixedIn &f,
             v.e2u_c = e2u[1];
             v.e2u_s = e2u[0];
             v.e2u_n = e2u[2];
-            v.e1u_c = dxu;
-            v.fcor = fcu;
-            v.area = aru;
-            v.ssha = sau;
+            v.e1u_c = dxu0;
+            v.fcor = fcu0;
+            v.area = aru0;
+            v.ssha = sau0;
             T val = T(0);
             bool wrote = in_rect(i, j, b.xs, b.xe - 1, b.ys, b.ye) &&
                          momentum_u_val(v, p, &val);
@@ -646,10 +685,10 @@ __device__ __forceinline__ void tile_mixed(const MixedIn &f,
             v.sv_w = from_below(sv[1]);
             v.hv_e = hv1e;
             v.sv_e = sv1e;
-            v.e2v_c = dyv;
-            v.fcor = fcv;
-            v.area = arv;
-            v.ssha = sav;
+            v.e2v_c = dyv0;
+            v.fcor = fcv0;
+            v.area = arv0;
+            v.ssha = sav0;
             T val = T(0);
             bool wrote = in_rect(i, j, b.xs, b.xe, b.ys, b.ye - 1) &&
                          momentum_v_val(v, p, &val);
@@ -664,25 +703,25 @@ __device__ __forceinline__ void tile_mixed(const MixedIn &f,
             const T at_e0 = from_above(at[0]);
             T val;
             if (owner && in_rect(i, j, b.xs, b.xe - 1, b.ys, b.ye) &&
-                next_ssh_pt(ssh_c.hi, ssh_e, at[0], at_e0, aru, tm[1], tm_e0, &val))
+                next_ssh_pt(ssh_c.hi, ssh_e, at[0], at_e0, aru0, tm[1], tm_e0, &val))
                 st32(o.sshu, c, val);
             if (owner && in_rect(i, j, b.xs, b.xe, b.ys, b.ye - 1) &&
-                next_ssh_pt(ssh_c.hi, ssh_n.hi, at[0], at[1], arv, tm[1], tm[2], &val))
+                next_ssh_pt(ssh_c.hi, ssh_n.hi, at[0], at[1], arv0, tm[1], tm[2], &val))
                 st32(o.sshv, c, val);
         }
 
-#define NEMO_SHIFT3(a) \
-    a[0] = a[1];       \
-    a[1] = a[2];
-#define NEMO_SHIFT2(a) a[0] = a[1];
-        NEMO_SHIFT3(un) NEMO_SHIFT3(vn) NEMO_SHIFT3(hu) NEMO_SHIFT3(su)
-        NEMO_SHIFT3(hv) NEMO_SHIFT3(sv) NEMO_SHIFT3(e1v) NEMO_SHIFT3(e2u)
-        NEMO_SHIFT2(ht) NEMO_SHIFT2(st) NEMO_SHIFT2(e1t) NEMO_SHIFT2(e2t) NEMO_SHIFT2(at)
-#undef NEMO_SHIFT3
-#undef NEMO_SHIFT2
-        tm[0] = tm[1];
-        tm[1] = tm[2];
-        tm[2] = tm[3];
+#define NEMO_SHIFT(a, n)                    \
+    _Pragma("unroll") for (int k = 0; k + 1 < (n); k++) a[k] = a[k + 1];
+        NEMO_SHIFT(un, 3 + PF) NEMO_SHIFT(vn, 3 + PF) NEMO_SHIFT(hu, 3 + PF)
+        NEMO_SHIFT(su, 3 + PF) NEMO_SHIFT(hv, 3 + PF) NEMO_SHIFT(sv, 3 + PF)
+        NEMO_SHIFT(e1v, 3 + PF) NEMO_SHIFT(e2u, 3 + PF)
+        NEMO_SHIFT(ht, 2 + PF) NEMO_SHIFT(st, 2 + PF) NEMO_SHIFT(e1t, 2 + PF)
+        NEMO_SHIFT(e2t, 2 + PF) NEMO_SHIFT(at, 2 + PF) NEMO_SHIFT(tm, 4 + PF)
+        NEMO_SHIFT(lo, 1 + PF)
+        NEMO_SHIFT(dxu, 1 + PF) NEMO_SHIFT(dyv, 1 + PF) NEMO_SHIFT(fcu, 1 + PF)
+        NEMO_SHIFT(fcv, 1 + PF) NEMO_SHIFT(aru, 1 + PF) NEMO_SHIFT(arv, 1 + PF)
+        NEMO_SHIFT(sau, 1 + PF) NEMO_SHIFT(sav, 1 + PF)
+#undef NEMO_SHIFT
         ssh_c = ssh_n;
         c_tm0e = tm_e0;
         c_tm1e = tm_e1;

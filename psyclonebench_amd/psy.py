@@ -124,7 +124,7 @@ class MixedStepper:
 
     FIELDS = {"sshn_t": 0, "un": 1, "vn": 2, "sshn_u": 3, "sshn_v": 4}
 
-    def __init__(self, model: Model, rows: int = 32, cols: int = 61):
+    def __init__(self, model: Model, rows: int = 32, cols: int = 61, prefetch: int = 0):
         lib = hip_lib()
         g = model.grid
         c = model.cfg
@@ -135,6 +135,7 @@ class MixedStepper:
                                          c.tide_omega)
         lib.nemo_mp_set_rows(self.handle, rows)
         lib.nemo_mp_set_columns(self.handle, cols)
+        lib.nemo_mp_set_prefetch(self.handle, prefetch)
 
     def run(self, nsteps: int) -> None:
         hip_lib().nemo_mp_steps(self.handle, nsteps, self.istp)

@@ -73,20 +73,23 @@ def test_mixed_rest_state_stays_zero():
 
 
 def test_mixed_band_shapes_bitwise_equal():
-    """Every rows-per-band and columns-per-wavefront shape of the mixed
-    kernel gives the same bits (509 columns: ragged last strips for 61, 56
-    and 48 columns)."""
-    cfg = c2_config(509, 1, seed=3)
+    """Every rows-per-band, columns-per-wavefront and prefetch shape of the
+    mixed kernel gives the same bits, on a 509^2 grid (ragged last strips
+    for 61, 56 and 48 columns) whose arrays are all random and nonzero, with
+    a random mask, over 8 steps."""
+    m = live_state_model(509, seed=54, random_mask=True)
+    shapes = [(rows, cols, 0) for cols in (61, 56, 48)
+              for rows in (4, 8, 16, 32, 48, 64, 80, 96, 128)]
+    shapes += [(rows, cols, 1) for cols in (61, 56, 48) for rows in (4, 8)]
     outs = []
-    for cols in (61, 56, 48):
-        for rows in (4, 8, 16, 32, 48, 64, 80, 96, 128):
-            st = MixedStepper(Model(cfg), rows=rows, cols=cols)
-            st.run(30)
-            outs.append({n: st.fetch(n) for n in ("sshn_t", "un", "vn", "sshn_u", "sshn_v")})
-            st.release()
-    for o in outs[1:]:
+    for rows, cols, pf in shapes:
+        st = MixedStepper(m.copy(), rows=rows, cols=cols, prefetch=pf)
+        st.run(8)
+        outs.append({n: st.fetch(n) for n in ("sshn_t", "un", "vn", "sshn_u", "sshn_v")})
+        st.release()
+    for (rows, cols, pf), o in zip(shapes[1:], outs[1:]):
         for n in o:
-            assert_bitwise(o[n], outs[0][n], n)
+            assert_bitwise(o[n], outs[0][n], f"{rows} rows, {cols} columns, prefetch {pf}: {n}")
 
 
 def test_mixed_graph_equals_timed_launches():
