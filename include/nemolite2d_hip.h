@@ -328,6 +328,15 @@ NEMO_API void nemo_cl_correct(void *handle, int guard, double *ssh, double *u, d
                               double *sshu, double *sshv, const double *ht,
                               const int *tmask, const double *area_t,
                               const double *area_u, const double *area_v);
+/* nsteps coarse steps istep, istep+1, ... of the coarse model whose ssha_t
+ * slot is ssha_t_device (stepped once before with c_invoke_time_step, fused
+ * form), each followed by the correction of nemo_cl_correct, as one hipGraph
+ * replay: the same results as alternating c_invoke_time_step and
+ * nemo_cl_correct. The model's slots are republished. A graph is captured
+ * on first use of (nsteps, parity, guard) and again after the
+ * normalisation, gate or guard settings change. */
+NEMO_API void nemo_cl_steps(void *handle, void **ssha_t_device, int nsteps, int istep,
+                            int guard);
 NEMO_API int nemo_cl_guard_trips(void *handle);
 /* HIP events around the forward pass of later calls; ms of the last one */
 NEMO_API void nemo_cl_set_timing(void *handle, int on);

@@ -182,6 +182,15 @@ class ClosureNet:
     def correct(self, coarse: TimeStepper, guard: bool = True) -> None:
         hip_lib().nemo_cl_correct(self.handle, 1 if guard else 0, *_state_ptrs(coarse))
 
+    def steps(self, coarse: TimeStepper, nsteps: int, guard: bool = True) -> None:
+        """nsteps coarse steps each followed by the correction, as one graph
+        replay (nemo_cl_steps); the same as alternating coarse.step() and
+        correct(coarse, guard)."""
+        slot = C.cast(C.byref(coarse.model.fields["ssha_t"].device_ptr),
+                      C.POINTER(C.c_void_p))
+        hip_lib().nemo_cl_steps(self.handle, slot, nsteps, coarse.istp, 1 if guard else 0)
+        coarse.istp += nsteps
+
     def guard_trips(self) -> int:
         return hip_lib().nemo_cl_guard_trips(self.handle)
 

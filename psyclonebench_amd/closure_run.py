@@ -317,16 +317,29 @@ def evaluate(n: int, factor: int, bathy: list[int], steps: int, every: int, weig
         free = free_cells(coarse)
         errs = {"plain": {f: [] for f in EVAL_FIELDS}, "learned": {f: [] for f in EVAL_FIELDS}}
         ratios = []
-        for k in range(steps):
-            sf.run(factor)
-            sp.run(1)
-            sl.run(1)
-            net.correct(sl, guard=True)
-            if trace_energy:
-                st = net.last_stats()
-                ratios.append((st["energy_after"] - st["energy_before"])
-                              / max(st["energy_before"], 1e-300))
-            if (k + 1) % every == 0:
+        k = 0
+        while k < steps:
+            # advance every run to the next comparison; the learned steps go
+            # as one graph replay of coarse step + correction (nemo_cl_steps)
+            # once the coarse model has its device state
+            nblk = min(every - k % every, steps - k)
+            if trace_energy or k == 0:
+                per_step = nblk if trace_energy else 1
+                for _ in range(per_step):
+                    sl.run(1)
+                    net.correct(sl, guard=True)
+                    if trace_energy:
+                        st = net.last_stats()
+                        ratios.append((st["energy_after"] - st["energy_before"])
+                                      / max(st["energy_before"], 1e-300))
+                if nblk > per_step:
+                    net.steps(sl, nblk - per_step, guard=True)
+            else:
+                net.steps(sl, nblk, guard=True)
+            sf.run(factor * nblk)
+            sp.run(nblk)
+            k += nblk
+            if k % every == 0:
                 net.filter_fine(sf, factor)
                 for f in FIELDS:
                     F = interior(net.filtered(f, (g.H, g.W)), g)
@@ -378,14 +391,10 @@ def throughput(n: int, factor: int, weights, steps: int = 200) -> dict:
     sf.run(factor * steps)
     hip_lib().nemo_hip_synchronize()
     fine_ms = (time.perf_counter() - t0) * 1e3 / steps
-    for _ in range(10):
-        sl.run(1)
-        net.correct(sl, guard=True)
+    net.steps(sl, steps, guard=True)      # captures the graph of `steps` learned steps
     hip_lib().nemo_hip_synchronize()
     t0 = time.perf_counter()
-    for _ in range(steps):
-        sl.run(1)
-        net.correct(sl, guard=True)
+    net.steps(sl, steps, guard=True)
     hip_lib().nemo_hip_synchronize()
     learned_ms = (time.perf_counter() - t0) * 1e3 / steps
     t0 = time.perf_counter()

@@ -40,6 +40,10 @@ struct ClosureContext {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timing = false;
     double last_ms = -1.0;
+    // changes whenever a value the correction kernels take as an argument
+    // changes (normalisation, gate, guard): graphs of nemo_cl_steps captured
+    // under an older generation are not replayed
+    unsigned long long generation = 0;
 
     size_t padded(int c) const { return (size_t)(ny + 4) * (size_t)(nx + 4) * (size_t)c; }
     float *bias(int l) const { return params + layer_offset(l) + layer_wsize(l); }
@@ -79,6 +83,13 @@ ClosureContext &ctx(void *h)
     return *static_cast<ClosureContext *>(h);
 }
 
+// one counter for every handle, so a generation is never reused
+unsigned long long next_generation()
+{
+    static unsigned long long g = 0;
+    return ++g;
+}
+
 void refresh_bf16(ClosureContext &C, hipStream_t s)
 {
     for (int l = 0; l < NLAYER; l++)
@@ -94,10 +105,11 @@ CoarseState state(ClosureContext &C, double *ssh, double *u, double *v, double *
 }
 
 // Forward pass from C.in0; training keeps the pre-activations; gated: the
-// output is multiplied by the activity gate k_pack computed with the input.
-void forward(ClosureContext &C, hipStream_t s, bool training, bool gated)
+// output is multiplied by the activity gate k_pack computed with the input;
+// timed: HIP events around it (C.timing, outside captured graphs).
+void forward(ClosureContext &C, hipStream_t s, bool training, bool gated, bool timed)
 {
-    if (C.timing) NEMO_HIP_CHECK(hipEventRecord(C.ev0, s));
+    if (timed) NEMO_HIP_CHECK(hipEventRecord(C.ev0, s));
     const __bf16 *x = C.in0;
     for (int l = 0; l < NLAYER; l++) {
         ConvArgs a{};
@@ -117,7 +129,39 @@ void forward(ClosureContext &C, hipStream_t s, bool training, bool gated)
             launch_conv(s, layer_cin(l), layer_cout(l), EPI_OUTPUT, a);
         }
     }
-    if (C.timing) NEMO_HIP_CHECK(hipEventRecord(C.ev1, s));
+    if (timed) NEMO_HIP_CHECK(hipEventRecord(C.ev1, s));
+}
+
+// The in-step correction of the coarse state c: pack the input, forward
+// pass, energy guard (guard != 0), corrected fields, face ssh.
+void enqueue_correction(ClosureContext &C, hipStream_t s, int guard, const CoarseState &c,
+                        bool timed)
+{
+    launch_pack(s, c, C.nm, C.in0, C.gate, C.nx, C.ny);
+    forward(C, s, false, true, timed);
+    if (guard) {
+        launch_guard(s, C.pred, C.nm, c, C.g, C.guard_partial, C.guard_rel, C.guard_abs,
+                     C.flag, C.trips, C.nx, C.ny);
+    }
+    const int *flag = guard ? C.flag : nullptr;
+    launch_set_fields(s, 1, nullptr, nullptr, nullptr, C.pred, C.nm, flag, c, C.nx, C.ny);
+    launch_face_ssh(s, flag, c, C.nx, C.ny);
+}
+
+struct CorrectionHook {
+    ClosureContext *C;
+    int guard;
+};
+
+void correction_hook(void *user, hipStream_t s, double *ssh, double *u, double *v,
+                     double *sshu, double *sshv, const double *ht, const int *tmask,
+                     const double *area_t, const double *area_u, const double *area_v)
+{
+    const CorrectionHook &h = *static_cast<const CorrectionHook *>(user);
+    enqueue_correction(*h.C, s, h.guard,
+                       CoarseState{ssh, u, v, sshu, sshv, ht, tmask, area_t, area_u, area_v,
+                                   h.C->Wc},
+                       false);
 }
 
 // Loss gradient already in C.dpre[NLAYER-1]; fills C.grads.
@@ -244,6 +288,7 @@ void *nemo_cl_create(int nx, int ny, int coarse_width, int coarse_height,
     C->trips = zeros<int>(1, s);
     NEMO_HIP_CHECK(hipEventCreate(&C->ev0));
     NEMO_HIP_CHECK(hipEventCreate(&C->ev1));
+    C->generation = next_generation();
     refresh_bf16(*C, s);
     NEMO_HIP_CHECK(hipStreamSynchronize(s));
     return C.release();
@@ -286,6 +331,7 @@ void nemo_cl_set_norm(void *h, const double *in_shift, const double *in_scale,
         C.nm.in_scale[k] = (float)in_scale[k];
     }
     for (int k = 0; k < 3; k++) C.nm.out_scale[k] = (float)out_scale[k];
+    C.generation = next_generation();
 }
 
 void nemo_cl_set_optimizer(void *h, int kind, double lr, double beta1, double beta2,
@@ -300,7 +346,12 @@ void nemo_cl_set_optimizer(void *h, int kind, double lr, double beta1, double be
     C.eps = (float)eps;
 }
 
-void nemo_cl_set_gate(void *h, double eps) { ctx(h).nm.gate_eps = (float)eps; }
+void nemo_cl_set_gate(void *h, double eps)
+{
+    ClosureContext &C = ctx(h);
+    C.nm.gate_eps = (float)eps;
+    C.generation = next_generation();
+}
 
 void nemo_cl_set_guard(void *h, double rel, double abs_tol, double g)
 {
@@ -308,6 +359,7 @@ void nemo_cl_set_guard(void *h, double rel, double abs_tol, double g)
     C.guard_rel = rel;
     C.guard_abs = abs_tol;
     C.g = g;
+    C.generation = next_generation();
 }
 
 void nemo_cl_filter(void *h, const double *fine_ssh, const double *fine_u,
@@ -347,7 +399,7 @@ void nemo_cl_train_step(void *h, double *ssh, double *u, double *v, double *sshu
     const CoarseState c = state(C, ssh, u, v, sshu, sshv, ht, tmask, area_t, area_u, area_v);
     hipStream_t s = lib_stream();
     launch_pack(s, c, C.nm, C.in0, C.gate, C.nx, C.ny);
-    forward(C, s, true, true);
+    forward(C, s, true, true, false);
     launch_target(s, C.fssh, C.fu, C.fv, c, C.nm, C.target, C.nx, C.ny);
     launch_loss(s, C.pred, C.target, C.dpre[NLAYER - 1], C.loss_partial, C.nx, C.ny, tmask,
                 C.Wc, C.gate, C.loss_w);
@@ -408,16 +460,15 @@ void nemo_cl_correct(void *h, int guard, double *ssh, double *u, double *v, doub
 {
     ClosureContext &C = ctx(h);
     const CoarseState c = state(C, ssh, u, v, sshu, sshv, ht, tmask, area_t, area_u, area_v);
-    hipStream_t s = lib_stream();
-    launch_pack(s, c, C.nm, C.in0, C.gate, C.nx, C.ny);
-    forward(C, s, false, true);
-    if (guard) {
-        launch_guard(s, C.pred, C.nm, c, C.g, C.guard_partial, C.guard_rel, C.guard_abs,
-                     C.flag, C.trips, C.nx, C.ny);
-    }
-    const int *flag = guard ? C.flag : nullptr;
-    launch_set_fields(s, 1, nullptr, nullptr, nullptr, C.pred, C.nm, flag, c, C.nx, C.ny);
-    launch_face_ssh(s, flag, c, C.nx, C.ny);
+    enqueue_correction(C, lib_stream(), guard, c, C.timing);
+}
+
+void nemo_cl_steps(void *h, void **ssha_t_device, int nsteps, int istep, int guard)
+{
+    ClosureContext &C = ctx(h);
+    CorrectionHook user{&C, guard ? 1 : 0};
+    const StepHook hook{&correction_hook, &user, 2ull * C.generation + (guard ? 1 : 0)};
+    psy_steps_with_hook(ssha_t_device, nsteps, istep, hook);
 }
 
 int nemo_cl_guard_trips(void *h)
@@ -445,7 +496,7 @@ void nemo_cl_forward_host(void *h, const float *in4, float *out3)
     ClosureContext &C = ctx(h);
     hipStream_t s = lib_stream();
     upload_inputs(C, in4, s);
-    forward(C, s, false, false);
+    forward(C, s, false, false, C.timing);
     const size_t npx = (size_t)C.nx * C.ny;
     std::vector<float> p(npx * 4);
     NEMO_HIP_CHECK(hipStreamSynchronize(s));
@@ -464,7 +515,7 @@ static void host_pair(ClosureContext &C, const float *in4, const float *target3,
         for (int c = 0; c < 3; c++) t[k * 4 + c] = target3[k * 3 + c];
     NEMO_HIP_CHECK(hipMemcpyAsync(C.target, t.data(), t.size() * sizeof(float),
                                   hipMemcpyHostToDevice, s));
-    forward(C, s, true, false);
+    forward(C, s, true, false, false);
     launch_loss(s, C.pred, C.target, C.dpre[NLAYER - 1], C.loss_partial, C.nx, C.ny,
                 nullptr, 0, nullptr, C.loss_w);
     backward(C, s);

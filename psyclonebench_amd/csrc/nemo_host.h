@@ -35,6 +35,26 @@ hipStream_t lib_stream();
 // (host libm, as boundary_conditions_kern.c:187-192).
 double tide_at(int istep, double rdt);
 
+// Work queued after every step of psy_steps_with_hook, on the step's stream,
+// over the state the step has just written: ssh, u, v, face ssh (the current
+// ping-pong buffers), and the constant ht, tmask and cell / face areas.
+struct StepHook {
+    void (*enqueue)(void *user, hipStream_t s, double *ssh, double *u, double *v,
+                    double *sshu, double *sshv, const double *ht, const int *tmask,
+                    const double *area_t, const double *area_u, const double *area_v);
+    void *user;
+    // identifies what enqueue queues (its buffers and the values it passes
+    // to kernels); graphs captured with one key are replayed for that key
+    unsigned long long key;
+};
+
+// nsteps fused steps istep, istep+1, ... of the device context whose ssha_t
+// slot is `ssha_t_slot` (made by c_invoke_time_step in the fused form), each
+// followed by hook.enqueue, as one hipGraph replay; the caller's slots are
+// republished as c_invoke_time_steps does. Exits if there is no such fused
+// context.
+void psy_steps_with_hook(void **ssha_t_slot, int nsteps, int istep, const StepHook &hook);
+
 template <typename T>
 inline T *dev_alloc(size_t n)
 {

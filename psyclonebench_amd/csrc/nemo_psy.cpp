@@ -67,7 +67,9 @@ struct Context {
     double *d_tide = nullptr;
     double *h_tide = nullptr;  // pinned
     int tide_cap = 0;
-    std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;  // n, parity, variant
+    // n, parity, variant, hook key (0: plain steps)
+    std::map<std::tuple<int, int, int, unsigned long long>, hipGraphExec_t> graphs;
+    void **slots[A_COUNT] = {};  // the caller's slots of the last call
     std::vector<hipEvent_t> ev;
 
     double *d(Arr a) const { return static_cast<double *>(buf[a]); }
@@ -243,10 +245,19 @@ void launch_fused_pair(Context &C, int par, hipStream_t s, const double *tide_ta
                    C.SU(par), C.SV(par));
 }
 
-void step_fused(Context &C, int istep, hipStream_t s)
+void run_hook(const Context &C, const StepHook *hook, hipStream_t s)
+{
+    if (!hook) return;
+    const int p = C.parity;
+    hook->enqueue(hook->user, s, C.T(p), C.U(p), C.V(p), C.SU(p), C.SV(p), C.d(A_HT),
+                  C.mask(), C.d(A_AREA_T), C.d(A_AREA_U), C.d(A_AREA_V));
+}
+
+void step_fused(Context &C, int istep, hipStream_t s, const StepHook *hook = nullptr)
 {
     launch_fused_pair(C, C.parity, s, nullptr, 0, tide_at(istep, C.P.rdt));
     C.parity = 1 - C.parity;
+    run_hook(C, hook, s);
     if (!C.paired) {
         // After the first step both sets must agree wherever no sweep
         // writes, so that later steps may write into either set.
@@ -275,14 +286,21 @@ void ensure_tide_table(Context &C, int n)
     C.graphs.clear();
 }
 
-hipGraphExec_t fused_graph(Context &C, int n, int par, hipStream_t s)
+hipGraphExec_t fused_graph(Context &C, int n, int par, hipStream_t s,
+                           const StepHook *hook)
 {
-    auto key = std::make_tuple(n, par, g_fused_variant);
+    auto key = std::make_tuple(n, par, g_fused_variant, hook ? hook->key : 0ull);
     auto it = C.graphs.find(key);
     if (it != C.graphs.end()) return it->second;
     hipGraph_t graph;
+    const int parity = C.parity;
     NEMO_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < n; k++) launch_fused_pair(C, (par + k) & 1, s, C.d_tide, k, 0.0);
+    for (int k = 0; k < n; k++) {
+        launch_fused_pair(C, (par + k) & 1, s, C.d_tide, k, 0.0);
+        C.parity = (par + k + 1) & 1;  // the set this step wrote, for the hook
+        run_hook(C, hook, s);
+    }
+    C.parity = parity;
     NEMO_HIP_CHECK(hipStreamEndCapture(s, &graph));
     hipGraphExec_t exec;
     NEMO_HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
@@ -291,15 +309,17 @@ hipGraphExec_t fused_graph(Context &C, int n, int par, hipStream_t s)
     return exec;
 }
 
-void run_steps(Context &C, int nsteps, int istep, hipStream_t s)
+void run_steps(Context &C, int nsteps, int istep, hipStream_t s,
+               const StepHook *hook = nullptr)
 {
     if (nsteps <= 0) return;
     if (C.mode == 1) {
+        if (hook) NEMO_FATAL("steps with a hook need the fused form");
         for (int k = 0; k < nsteps; k++) step_unfused(C, istep + k, s);
         return;
     }
     if (!C.paired) {  // first step sets up the pairing
-        step_fused(C, istep, s);
+        step_fused(C, istep, s, hook);
         istep++;
         nsteps--;
         if (nsteps == 0) return;
@@ -317,9 +337,14 @@ void run_steps(Context &C, int nsteps, int istep, hipStream_t s)
                 C.ev.push_back(e);
             }
         }
-        for (int k = 0; k < nsteps; k++)
-            launch_fused_pair(C, (C.parity + k) & 1, s, C.d_tide, k, 0.0,
-                              C.ev[2 * k], C.ev[2 * k + 1]);
+        const int parity = C.parity;
+        for (int k = 0; k < nsteps; k++) {
+            launch_fused_pair(C, (parity + k) & 1, s, C.d_tide, k, 0.0, C.ev[2 * k],
+                              C.ev[2 * k + 1]);
+            C.parity = (parity + k + 1) & 1;
+            run_hook(C, hook, s);
+        }
+        C.parity = parity;
         NEMO_HIP_CHECK(hipStreamSynchronize(s));
         double sum = 0.0;
         for (int k = 0; k < nsteps; k++) {
@@ -329,7 +354,7 @@ void run_steps(Context &C, int nsteps, int istep, hipStream_t s)
         }
         g_fused_ms = sum / nsteps;
     } else {
-        NEMO_HIP_CHECK(hipGraphLaunch(fused_graph(C, nsteps, C.parity, s), s));
+        NEMO_HIP_CHECK(hipGraphLaunch(fused_graph(C, nsteps, C.parity, s, hook), s));
     }
     C.parity = (C.parity + nsteps) & 1;
 }
@@ -427,6 +452,24 @@ static Context &device_context(double *const host[A_COUNT], void **slot[A_COUNT]
     return C;
 }
 
+namespace nemo {
+
+void psy_steps_with_hook(void **ssha_t_slot, int nsteps, int istep, const StepHook &hook)
+{
+    auto it = g_contexts.find(ssha_t_slot);
+    if (it == g_contexts.end())
+        NEMO_FATAL("steps with a hook: no device context for these slots (step the model "
+                   "once with c_invoke_time_step first)");
+    Context &C = *it->second;
+    if (C.mode != 2) NEMO_FATAL("steps with a hook need the fused form");
+    if (nsteps < 0) NEMO_FATAL("nsteps must be >= 0, got %d", nsteps);
+    if (g_kernel_timing) g_fused_ms = -1.0;
+    run_steps(C, nsteps, istep, lib_stream(), &hook);
+    publish(C, C.slots);
+}
+
+}  // namespace nemo
+
 extern "C" {
 
 void c_invoke_time_step(NEMO_HOST_ARRAYS, NEMO_DEVICE_SLOTS, int istep,
@@ -461,6 +504,7 @@ void c_invoke_time_steps(int nsteps, NEMO_HOST_ARRAYS, NEMO_DEVICE_SLOTS,
                      internal_ystop, width, total_size};
     const Physics P{rdt, cbfr, visc, omega, d2r, g};
     Context &C = device_context(host, slot, G, P);
+    for (int a = 0; a < A_COUNT; a++) C.slots[a] = slot[a];
     if (nsteps < 0) NEMO_FATAL("nsteps must be >= 0, got %d", nsteps);
     if (g_kernel_timing && C.mode == 2) g_fused_ms = -1.0;
     hipStream_t s = lib_stream();

@@ -263,3 +263,37 @@ def test_training_command_end_to_end(tmp_path):
     res = cr.evaluate(256, 4, [2], 20, 10, out, log_fn=lambda *x: None)
     for f in ("sshn_t", "un", "vn"):
         assert math.isfinite(res["ratio_mean"][f]), f
+
+
+@pytest.mark.parametrize("guard", [True, False])
+def test_captured_learned_steps_equal_stepwise(guard):
+    """nemo_cl_steps (coarse step + correction, captured as one graph and
+    replayed) against alternating c_invoke_time_step and nemo_cl_correct on
+    a twin model: bitwise equal after 7 + 30 steps (two graph lengths, the
+    second from the other ping-pong parity), with the same guard trips. A
+    settings change (the guard) must be seen by the next replay."""
+    _, coarse = _fine_and_coarse()
+    twin = coarse.copy()
+    a, b = TimeStepper(coarse, mode=2), TimeStepper(twin, mode=2)
+    a.run(3)
+    b.run(3)
+    na = ClosureNet(coarse, params=net_params(5))
+    nb = ClosureNet(twin, params=net_params(5))
+    for net in (na, nb):
+        net.set_norm([0, 0, 0, 100], [0.5, 0.5, 0.5, 50], [1e-3, 1e-3, 1e-3])
+        net.set_gate(0.01)
+        net.set_guard(0.05, 1e-3)
+    fields = ("sshn_t", "un", "vn", "sshn_u", "sshn_v")
+    for n, rel_guard, abs_tol in ((7, 0.05, 1e-3), (30, 0.05, 1e-3), (30, 1e-9, 0.0)):
+        for net in (na, nb):
+            net.set_guard(rel_guard, abs_tol)
+        na.steps(a, n, guard=guard)
+        for _ in range(n):
+            b.step()
+            nb.correct(b, guard=guard)
+        for f in fields:
+            assert_bitwise(a.fetch(f), b.fetch(f), f"{n} steps, guard {rel_guard}: {f}")
+        assert na.guard_trips() == nb.guard_trips()
+    print("guard trips:", na.guard_trips(), flush=True)
+    na.release()
+    nb.release()
