@@ -286,6 +286,12 @@ NEMO_API void nemo_cl_set_norm(void *handle, const double *in_shift,
  * 1 - exp(-(a/eps)^2), a = |u| + |v| + |ssh| in normalised input units, so
  * points at rest get none (eps <= 0: no gate; default 0) */
 NEMO_API void nemo_cl_set_gate(void *handle, double eps);
+/* ssh cells the correction changes and the training loss counts: 1 every wet
+ * interior cell, including the ones the step clamps to the tide (wet cells
+ * with an open 4-neighbour, whose next step starts from the corrected value
+ * and clamps them again); 0 (default) only the cells the step does not
+ * clamp */
+NEMO_API void nemo_cl_set_ssh_cells(void *handle, int all_wet);
 /* kind 0: SGD with momentum beta1; kind 1: Adam */
 NEMO_API void nemo_cl_set_optimizer(void *handle, int kind, double lr, double beta1,
                                     double beta2, double eps);

@@ -167,6 +167,11 @@ class ClosureNet:
     def set_gate(self, eps: float) -> None:
         hip_lib().nemo_cl_set_gate(self.handle, eps)
 
+    def set_ssh_cells(self, all_wet: bool) -> None:
+        """Correct and learn the ssh of every wet cell (True) or only of the
+        cells the step does not clamp to the tide (False)."""
+        hip_lib().nemo_cl_set_ssh_cells(self.handle, 1 if all_wet else 0)
+
     def set_clip(self, max_norm: float) -> None:
         hip_lib().nemo_cl_set_clip(self.handle, max_norm)
 

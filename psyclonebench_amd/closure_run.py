@@ -91,9 +91,12 @@ def guard_abs(coarse: Model) -> float:
 
 
 def configure(net: ClosureNet, norm: dict) -> None:
-    """Normalisation and activity gate of a trained network."""
+    """Normalisation, activity gate and corrected ssh cells of a trained
+    network (weights saved before the ssh-cell setting existed correct the
+    free cells only)."""
     net.set_norm(norm["in_shift"], norm["in_scale"], norm["out_scale"])
     net.set_gate(norm.get("gate_eps", 0.0))
+    net.set_ssh_cells(norm.get("ssh_cells", "free") == "wet")
 
 
 def free_cells(coarse: Model) -> np.ndarray:
@@ -109,10 +112,17 @@ def _active_rms(a: np.ndarray) -> float:
     return float(np.sqrt(np.mean(a[m] ** 2))) if np.any(m) else 1.0
 
 
-def pilot_norm(n: int, factor: int, seed: int, steps: int, every: int = 50) -> dict:
+def wet_cells(coarse: Model) -> np.ndarray:
+    """Interior T cells that are wet."""
+    return interior(coarse.array("tmask") > 0, coarse.grid)
+
+
+def pilot_norm(n: int, factor: int, seed: int, steps: int, every: int = 50,
+               ssh_cells: str = "free") -> dict:
     """Input and target scales from a plain run of one seed (each coarse step
     started from the filtered fine state): RMS over the cells where the field
-    is not (numerically) zero, sampled every `every` coarse steps."""
+    is not (numerically) zero, sampled every `every` coarse steps; the ssh
+    over the cells the closure corrects (ssh_cells "wet" or "free")."""
     fine, coarse = pair(n, factor, seed, steps)
     sf, sc = TimeStepper(fine, mode=2), TimeStepper(coarse, mode=2)
     net = ClosureNet(coarse)
@@ -133,11 +143,10 @@ def pilot_norm(n: int, factor: int, seed: int, steps: int, every: int = 50) -> d
     net.release()
     sf.release()
     sc.release()
-    # the ssh of cells the step clamps to the tide is neither corrected nor
-    # learned: leave it out of the ssh scales
-    free = free_cells(coarse)
-    xs["sshn_t"] = [a[free] for a in xs["sshn_t"]]
-    ts["sshn_t"] = [a[free] for a in ts["sshn_t"]]
+    # the ssh scales cover the cells whose ssh is corrected and learned
+    cells = wet_cells(coarse) if ssh_cells == "wet" else free_cells(coarse)
+    xs["sshn_t"] = [a[cells] for a in xs["sshn_t"]]
+    ts["sshn_t"] = [a[cells] for a in ts["sshn_t"]]
     ht = interior(coarse.array("ht"), g)
     order = ("un", "vn", "sshn_t")
     return {"in_shift": [0.0, 0.0, 0.0, float(np.mean(ht))],
@@ -157,7 +166,8 @@ def train(a) -> None:
     else:
         params = init_params(a.seed)
         log("pilot run for the normalisation, seed", bathy[0])
-        norm = pilot_norm(a.n, a.factor, bathy[0], a.pilot_steps)
+        norm = pilot_norm(a.n, a.factor, bathy[0], a.pilot_steps, ssh_cells=a.ssh_cells)
+        norm["ssh_cells"] = a.ssh_cells
     if a.gate_eps is not None:
         norm["gate_eps"] = a.gate_eps
     log("norm", json.dumps(norm))
@@ -262,20 +272,33 @@ def train(a) -> None:
 
 def validation_score(a, params, norm, history) -> tuple[float, dict, int]:
     """Model selection: the learned / plain error ratio of `params` on the
-    held-out --val-bathy seed, over --val-steps coarse steps from rest (mean
-    over time, averaged over u, v and the free-cell ssh), plus the fraction
-    of those steps whose correction the energy guard rejected."""
+    held-out --val-bathy seeds, over --val-steps coarse steps from rest (mean
+    over time and seeds, averaged over u, v and the ssh of the cells the
+    closure corrects; the worst seed's average weighs in as much as the
+    mean), plus the fraction of those steps whose correction the energy
+    guard rejected."""
     tmp = Path(a.out).with_name(Path(a.out).stem + ".validation.npz")
     save(tmp, params, norm, a, history)
+    vseeds = val_seeds(a.val_bathy)
     try:
-        res = evaluate(a.n, a.factor, [a.val_bathy], a.val_steps,
+        res = evaluate(a.n, a.factor, vseeds, a.val_steps,
                        max(a.val_steps // 10, 1), tmp, log_fn=lambda *x: None)
     finally:
         tmp.unlink()
-    seed = res["seeds"][a.val_bathy]
-    ratios = {f: float(res["ratio_mean"][f]) for f in ("un", "vn", "sshn_t_free")}
-    trips = int(seed["guard_trips"])
-    return float(np.mean(list(ratios.values()))) + trips / a.val_steps, ratios, trips
+    ssh = "sshn_t" if norm.get("ssh_cells", "free") == "wet" else "sshn_t_free"
+    fields = ("un", "vn", ssh)
+    ratios = {f: float(res["ratio_mean"][f]) for f in fields}
+    worst = max(float(np.mean([res["seeds"][b]["ratio_mean"][f] for f in fields]))
+                for b in vseeds)
+    trips = sum(int(res["seeds"][b]["guard_trips"]) for b in vseeds)
+    score = 0.5 * (float(np.mean(list(ratios.values()))) + worst)
+    return score + trips / (a.val_steps * len(vseeds)), ratios, trips
+
+
+def val_seeds(spec) -> list[int]:
+    """--val-bathy: one seed ("1000") or a range ("1000:1002")."""
+    spec = str(spec)
+    return seeds(spec) if ":" in spec else [int(spec)]
 
 
 def save(path, params, norm, a, history, extra=None) -> None:
@@ -453,13 +476,18 @@ def main(argv=None) -> None:
     t.add_argument("--reset-spike", type=float, default=4.0,
                    help="restart a rollout from the filtered fine state when a step's loss "
                         "exceeds this multiple of the seed's running mean (0: never)")
-    t.add_argument("--val-bathy", type=int, default=1000,
-                   help="held-out bathymetry seed of the model-selection runs")
+    t.add_argument("--val-bathy", default="1000:1002",
+                   help="held-out bathymetry seed(s) of the model-selection runs: one seed "
+                        "or a range a:b")
     t.add_argument("--val-every", type=int, default=8,
                    help="training seeds between model-selection runs (0: keep the last "
                         "weights)")
-    t.add_argument("--val-steps", type=int, default=1000,
-                   help="coarse steps of each model-selection run")
+    t.add_argument("--val-steps", type=int, default=2000,
+                   help="coarse steps of each model-selection run (as long as the "
+                        "evaluation)")
+    t.add_argument("--ssh-cells", choices=("wet", "free"), default="wet",
+                   help="ssh cells the closure corrects and learns: every wet cell, or "
+                        "only the cells the step does not clamp to the tide")
     t.add_argument("--resume", default=None)
     t.add_argument("--out", default=str(DEFAULT_WEIGHTS))
     e = sub.add_parser("eval")

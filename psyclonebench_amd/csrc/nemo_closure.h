@@ -49,6 +49,9 @@ struct Norm {
     // 1 - exp(-(a / gate_eps)^2), a = |u| + |v| + |ssh| in input units, so
     // that a point at rest gets no correction (gate_eps <= 0: no gate).
     float gate_eps;
+    // ssh cells corrected and learned: 1 every wet cell, 0 only the cells
+    // the step does not clamp to the tide (wet, no open 4-neighbour)
+    int ssh_wet;
 };
 
 // ---- network kernels -------------------------------------------------------
@@ -102,10 +105,10 @@ struct LossWeights {
 // Weighted MSE loss over the 3 used channels, sum_c w_c sum_p d_c^2 / (3 npx):
 // gradient into dpre4 (COUT4 channels, 16-channel plane) and per-block
 // partial sums of the unweighted squared error per channel. With a mask
-// (pitch W), the ssh of cells the step clamps to the tide is left out.
+// (pitch W), the ssh of cells not corrected (Norm::ssh_wet) is left out.
 int loss_blocks();
 void launch_loss(hipStream_t s, const float *pred, const float *target, __bf16 *dpre4,
-                 float *partial, int nx, int ny, const int *tmask, int W,
+                 float *partial, int nx, int ny, const int *tmask, int W, int ssh_wet,
                  const float *gate, const LossWeights &lw);
 
 // ---- fine <-> coarse plumbing ----------------------------------------------

@@ -783,8 +783,7 @@ __global__ void k_target(const double *__restrict__ fssh, const double *__restri
 }
 
 // A wet T cell whose ssh the step does not clamp to the tide (bc_ssh sets
-// every wet cell with an open (-1) 4-neighbour): the closure corrects and
-// learns the ssh of these cells only.
+// every wet cell with an open (-1) 4-neighbour).
 constexpr int GUARD_PPT = 4;  // guard points per thread and pass
 
 __device__ __forceinline__ bool free_ssh_cell(const int *tm, size_t k, int W)
@@ -792,11 +791,19 @@ __device__ __forceinline__ bool free_ssh_cell(const int *tm, size_t k, int W)
     return tm[k] > 0 && tm[k - 1] >= 0 && tm[k + 1] >= 0 && tm[k - W] >= 0 && tm[k + W] >= 0;
 }
 
+// A cell whose ssh the closure corrects and learns: every wet cell
+// (ssh_wet), or the free cells only.
+__device__ __forceinline__ bool ssh_cell(const int *tm, size_t k, int W, int ssh_wet)
+{
+    return ssh_wet ? tm[k] > 0 : free_ssh_cell(tm, k, W);
+}
+
 __global__ __launch_bounds__(256) void k_loss(const float *__restrict__ pred,
                                               const float *__restrict__ target,
                                               __bf16 *__restrict__ dpre4,
                                               float *__restrict__ partial, int nx, int ny,
                                               const int *__restrict__ tmask, int W,
+                                              int ssh_wet,
                                               const float *__restrict__ gate, LossWeights lw)
 {
     __shared__ float red[3][256];
@@ -809,7 +816,7 @@ __global__ __launch_bounds__(256) void k_loss(const float *__restrict__ pred,
         const float4 b = reinterpret_cast<const float4 *>(target)[p];
         const int y = p / nx, x = p - y * nx;
         const bool ssh_free =
-            !tmask || free_ssh_cell(tmask, (size_t)(y + 1) * W + (x + 1), W);
+            !tmask || ssh_cell(tmask, (size_t)(y + 1) * W + (x + 1), W, ssh_wet);
         const float d0 = a.x - b.x, d1 = a.y - b.y, d2 = ssh_free ? a.z - b.z : 0.0f;
         s0 += d0 * d0;
         s1 += d1 * d1;
@@ -871,7 +878,8 @@ __global__ void k_set_fields(int mode, const double *__restrict__ sssh,
     float4 q = {0.f, 0.f, 0.f, 0.f};
     if (mode == 1) q = reinterpret_cast<const float4 *>(pred)[p];
     if (mode == 0 && wet) c.ssh[k] = sssh[k];
-    if (mode == 1 && free_ssh_cell(c.tmask, k, c.W)) c.ssh[k] += (double)(q.z * nm.out_scale[2]);
+    if (mode == 1 && ssh_cell(c.tmask, k, c.W, nm.ssh_wet))
+        c.ssh[k] += (double)(q.z * nm.out_scale[2]);
     if (I < nx && wet && c.tmask[k + 1] > 0) {
         if (mode == 0) c.u[k] = su[k];
         else c.u[k] += (double)(q.x * nm.out_scale[0]);
@@ -921,7 +929,7 @@ __global__ __launch_bounds__(256) void k_guard_partial(const float *__restrict__
             const double s = c.ssh[k], u = c.u[k], v = c.v[k], h = c.ht[k];
             const bool wet = c.tmask[k] > 0;
             const double s1 =
-                free_ssh_cell(c.tmask, k, c.W) ? s + (double)(q.z * nm.out_scale[2]) : s;
+                ssh_cell(c.tmask, k, c.W, nm.ssh_wet) ? s + (double)(q.z * nm.out_scale[2]) : s;
             const double u1 = (I < nx && wet && c.tmask[k + 1] > 0)
                                   ? u + (double)(q.x * nm.out_scale[0]) : u;
             const double v1 = (J < ny && wet && c.tmask[k + c.W] > 0)
@@ -1113,12 +1121,12 @@ void launch_target(hipStream_t s, const double *fssh, const double *fu, const do
 int loss_blocks() { return 256; }
 
 void launch_loss(hipStream_t s, const float *pred, const float *target, __bf16 *dpre4,
-                 float *partial, int nx, int ny, const int *tmask, int W,
+                 float *partial, int nx, int ny, const int *tmask, int W, int ssh_wet,
                  const float *gate, const LossWeights &lw)
 {
     k_loss<<<dim3((unsigned)loss_blocks()), dim3(256), 0, s>>>(pred, target, dpre4,
                                                                partial, nx, ny, tmask, W,
-                                                               gate, lw);
+                                                               ssh_wet, gate, lw);
     NEMO_LAUNCH_CHECK();
 }
 

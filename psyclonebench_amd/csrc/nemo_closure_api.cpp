@@ -346,6 +346,13 @@ void nemo_cl_set_optimizer(void *h, int kind, double lr, double beta1, double be
     C.eps = (float)eps;
 }
 
+void nemo_cl_set_ssh_cells(void *h, int all_wet)
+{
+    ClosureContext &C = ctx(h);
+    C.nm.ssh_wet = all_wet ? 1 : 0;
+    C.generation = next_generation();
+}
+
 void nemo_cl_set_gate(void *h, double eps)
 {
     ClosureContext &C = ctx(h);
@@ -402,7 +409,7 @@ void nemo_cl_train_step(void *h, double *ssh, double *u, double *v, double *sshu
     forward(C, s, true, true, false);
     launch_target(s, C.fssh, C.fu, C.fv, c, C.nm, C.target, C.nx, C.ny);
     launch_loss(s, C.pred, C.target, C.dpre[NLAYER - 1], C.loss_partial, C.nx, C.ny, tmask,
-                C.Wc, C.gate, C.loss_w);
+                C.Wc, C.nm.ssh_wet, C.gate, C.loss_w);
     backward(C, s);
     optimizer_step(C, s);
 }
@@ -517,7 +524,7 @@ static void host_pair(ClosureContext &C, const float *in4, const float *target3,
                                   hipMemcpyHostToDevice, s));
     forward(C, s, true, false, false);
     launch_loss(s, C.pred, C.target, C.dpre[NLAYER - 1], C.loss_partial, C.nx, C.ny,
-                nullptr, 0, nullptr, C.loss_w);
+                nullptr, 0, 0, nullptr, C.loss_w);
     backward(C, s);
     NEMO_HIP_CHECK(hipStreamSynchronize(s));  // t goes out of scope
 }
