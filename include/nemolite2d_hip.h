@@ -298,6 +298,11 @@ NEMO_API void nemo_cl_set_optimizer(void *handle, int kind, double lr, double be
 /* correction kept when finite and E_after <= (1 + rel) E_before + abs_tol,
  * E = sum over interior cells of g ssh^2 + ht (u^2 + v^2) */
 NEMO_API void nemo_cl_set_guard(void *handle, double rel, double abs_tol, double g);
+/* the guard also rejects a correction with any applied component beyond
+ * max_out in normalised output units, however the energy changes (a few
+ * large local corrections fit a global energy budget and can wreck the next
+ * steps); 0 (default): no such bound */
+NEMO_API void nemo_cl_set_guard_limit(void *handle, double max_out);
 /* box filter of a fine state (device pointers, pitch fine_width) by
  * `factor` onto the coarse grid, kept in the handle */
 NEMO_API void nemo_cl_filter(void *handle, const double *fine_ssh, const double *fine_u,

@@ -148,6 +148,7 @@ def hip_lib() -> C.CDLL:
         lib.nemo_cl_set_loss_weights.argtypes = [vp] + [C.c_double] * 3
         lib.nemo_cl_set_gate.argtypes = [vp, C.c_double]
         lib.nemo_cl_set_ssh_cells.argtypes = [vp, C.c_int]
+        lib.nemo_cl_set_guard_limit.argtypes = [vp, C.c_double]
         lib.nemo_cl_last_stats.argtypes = [vp, _dp]
         lib.nemo_cl_correct.argtypes = [vp, C.c_int] + st
         lib.nemo_cl_steps.argtypes = [vp, _vpp, C.c_int, C.c_int, C.c_int]

@@ -167,6 +167,11 @@ class ClosureNet:
     def set_gate(self, eps: float) -> None:
         hip_lib().nemo_cl_set_gate(self.handle, eps)
 
+    def set_guard_limit(self, max_out: float) -> None:
+        """Reject corrections with any component beyond max_out normalised
+        units as well (0: no bound)."""
+        hip_lib().nemo_cl_set_guard_limit(self.handle, max_out)
+
     def set_ssh_cells(self, all_wet: bool) -> None:
         """Correct and learn the ssh of every wet cell (True) or only of the
         cells the step does not clamp to the tide (False)."""

@@ -97,6 +97,7 @@ def configure(net: ClosureNet, norm: dict) -> None:
     net.set_norm(norm["in_shift"], norm["in_scale"], norm["out_scale"])
     net.set_gate(norm.get("gate_eps", 0.0))
     net.set_ssh_cells(norm.get("ssh_cells", "free") == "wet")
+    net.set_guard_limit(norm.get("guard_max_out", 0.0))
 
 
 def free_cells(coarse: Model) -> np.ndarray:
@@ -168,6 +169,7 @@ def train(a) -> None:
         log("pilot run for the normalisation, seed", bathy[0])
         norm = pilot_norm(a.n, a.factor, bathy[0], a.pilot_steps, ssh_cells=a.ssh_cells)
         norm["ssh_cells"] = a.ssh_cells
+    norm["guard_max_out"] = a.guard_max_out
     if a.gate_eps is not None:
         norm["gate_eps"] = a.gate_eps
     log("norm", json.dumps(norm))
@@ -252,6 +254,8 @@ def train(a) -> None:
                                 "guard_trips": trips})
             log(f"validation after seed {b}: score {score:.3f}  ratios "
                 f"{ {k: round(v, 3) for k, v in ratios.items()} }  trips {trips}")
+            if not np.isfinite(score):
+                score = float("inf")  # a run that went non-finite is the worst
             if best is None or score < best[0]:
                 best = (score, params.copy(), b)
     net.release()
@@ -485,7 +489,10 @@ def main(argv=None) -> None:
     t.add_argument("--val-steps", type=int, default=2000,
                    help="coarse steps of each model-selection run (as long as the "
                         "evaluation)")
-    t.add_argument("--ssh-cells", choices=("wet", "free"), default="wet",
+    t.add_argument("--guard-max-out", type=float, default=20.0,
+                   help="the energy guard also rejects corrections with any component "
+                        "beyond this many normalised units (0: no bound)")
+    t.add_argument("--ssh-cells", choices=("wet", "free"), default="free",
                    help="ssh cells the closure corrects and learns: every wet cell, or "
                         "only the cells the step does not clamp to the tide")
     t.add_argument("--resume", default=None)

@@ -52,6 +52,9 @@ struct Norm {
     // ssh cells corrected and learned: 1 every wet cell, 0 only the cells
     // the step does not clamp to the tide (wet, no open 4-neighbour)
     int ssh_wet;
+    // the energy guard also rejects a correction with any applied component
+    // beyond max_out in output units (0: no such bound)
+    float max_out;
 };
 
 // ---- network kernels -------------------------------------------------------
@@ -131,6 +134,7 @@ void launch_face_ssh(hipStream_t s, const int *flag, const CoarseState &c, int n
 // before and after the correction, and a non-finite count, as per-workgroup
 // partial sums (partial[block][3]); a second launch sets
 // flag = finite && E_after <= (1 + rel) * E_before + abs and counts trips.
+// A component beyond Norm::max_out counts as non-finite.
 int guard_blocks();
 void launch_guard(hipStream_t s, const float *pred, const Norm &nm, const CoarseState &c,
                   double g, double *partial, double rel, double abs_tol, int *flag,

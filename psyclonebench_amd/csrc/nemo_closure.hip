@@ -928,12 +928,16 @@ __global__ __launch_bounds__(256) void k_guard_partial(const float *__restrict__
             const float4 q = reinterpret_cast<const float4 *>(pred)[p];
             const double s = c.ssh[k], u = c.u[k], v = c.v[k], h = c.ht[k];
             const bool wet = c.tmask[k] > 0;
-            const double s1 =
-                ssh_cell(c.tmask, k, c.W, nm.ssh_wet) ? s + (double)(q.z * nm.out_scale[2]) : s;
-            const double u1 = (I < nx && wet && c.tmask[k + 1] > 0)
-                                  ? u + (double)(q.x * nm.out_scale[0]) : u;
-            const double v1 = (J < ny && wet && c.tmask[k + c.W] > 0)
-                                  ? v + (double)(q.y * nm.out_scale[1]) : v;
+            const bool cs = ssh_cell(c.tmask, k, c.W, nm.ssh_wet);
+            const bool cu = I < nx && wet && c.tmask[k + 1] > 0;
+            const bool cv = J < ny && wet && c.tmask[k + c.W] > 0;
+            const double s1 = cs ? s + (double)(q.z * nm.out_scale[2]) : s;
+            const double u1 = cu ? u + (double)(q.x * nm.out_scale[0]) : u;
+            const double v1 = cv ? v + (double)(q.y * nm.out_scale[1]) : v;
+            if (nm.max_out > 0.0f && ((cs && fabsf(q.z) > nm.max_out) ||
+                                      (cu && fabsf(q.x) > nm.max_out) ||
+                                      (cv && fabsf(q.y) > nm.max_out)))
+                bad += 1.0;
             e0 += g * s * s + h * (u * u + v * v);
             const double t = g * s1 * s1 + h * (u1 * u1 + v1 * v1);
             if (isfinite(t)) e1 += t;

@@ -346,6 +346,14 @@ void nemo_cl_set_optimizer(void *h, int kind, double lr, double beta1, double be
     C.eps = (float)eps;
 }
 
+void nemo_cl_set_guard_limit(void *h, double max_out)
+{
+    ClosureContext &C = ctx(h);
+    if (!(max_out >= 0.0)) NEMO_FATAL("closure: guard limit must be >= 0 (got %g)", max_out);
+    C.nm.max_out = (float)max_out;
+    C.generation = next_generation();
+}
+
 void nemo_cl_set_ssh_cells(void *h, int all_wet)
 {
     ClosureContext &C = ctx(h);
