@@ -40,6 +40,7 @@ B_ALG_F64 = 212          # bytes per interior grid-point update, FP64
 B_ALG_MIXED = 113        # FP32 fields, the byte mask and the ssh compensation word
 # rocprofv3 summaries holding "[label] traffic_bytes_per_launch:" lines
 PROFILE_TRAFFIC = (ROOT / "profiles" / "fused_step_4096_counters.md",
+                   ROOT / "profiles" / "ensemble_step_256_domains_counters.md",
                    ROOT / "profiles" / "step_kernels_4096_traffic.md")
 
 
@@ -69,7 +70,7 @@ def parse_args():
 
 def traffic_from_profile(label: str):
     """HBM bytes per launch of the kernel recorded under `label` ("variant 13",
-    "mixed rows 64"), measured with rocprofv3 PMC counters (FETCH_SIZE +
+    "mixed rows 64", "ensemble f64 4x56p1"), measured with rocprofv3 PMC counters (FETCH_SIZE +
     WRITE_SIZE, scaled by the copy-kernel calibration) and kept in
     profiles/; None when that kernel shape was not profiled."""
     key = f"[{label}] traffic_bytes_per_launch:"
@@ -286,13 +287,18 @@ def ensemble_leg(a) -> dict:
             "unit": "grid-pt updates/s",
             "interior_points": pts,
             "ms_per_step": wall * 1e3 / a.ens_steps,
-            "shape": {"rows": shape[0], "columns": shape[3],
-                      "waves_per_workgroup": shape[1], "workgroups": shape[2]},
+            "shape": {"rows": shape[0], "columns": shape[3], "prefetch": shape[4],
+                      "launch": "persistent" if shape[2] else "one wavefront per tile",
+                      "waves_per_workgroup": shape[1], "workgroups": shape[2] or None},
             "autotune_kernel_ms": [round(x, 4) for x in tune_ms],
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                         "traffic": (traffic_from_profile(
+                             f"ensemble f{prec} {shape[0]}x{shape[3]}p{shape[4]}")
+                             if shape[2] == 0 else None),
                          "kernel": (f"k_ens_{'f64' if prec == 64 else 'mixed'}"
-                                    f"<{shape[0]}, {shape[3]}>"),
+                                    f"{'' if shape[2] else '_grid'}"
+                                    f"<{shape[0]}, {shape[3]}, {shape[4]}>"),
                          "kernel_ms": kernel_ms, "bytes_per_launch_alg": b_alg * pts},
         }
         ens.release()

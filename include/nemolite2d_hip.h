@@ -239,12 +239,17 @@ NEMO_API void *nemo_ens_create(int ndom, void *const *arrays, const int *geom,
                                double rdt, double cbfr, double visc, double omega,
                                double d2r, double g, int precision);
 /* Launch shapes: rows per band (tile height), wavefronts per workgroup,
- * workgroups per launch; nemo_ens_shape_columns gives the tile width (owned
- * columns per wavefront: 61, or 56 for 64-byte-aligned stores). */
+ * workgroups per launch (0: one one-wavefront workgroup per tile; else a
+ * persistent launch of that many); nemo_ens_shape_columns gives the tile
+ * width (owned columns per wavefront: 61, or 56 for 64-byte-aligned stores,
+ * or 0: per domain, whichever of the two its width runs faster with) and
+ * nemo_ens_shape_prefetch the rows of loads the tile issues ahead of the row
+ * that needs them, beyond the next one (0 or 1). */
 NEMO_API int nemo_ens_shape_count(void *handle);
 NEMO_API void nemo_ens_shape_info(void *handle, int shape, int *rows,
                                   int *waves_per_block, int *blocks);
 NEMO_API int nemo_ens_shape_columns(void *handle, int shape);
+NEMO_API int nemo_ens_shape_prefetch(void *handle, int shape);
 NEMO_API void nemo_ens_set_shape(void *handle, int shape);
 NEMO_API int nemo_ens_shape(void *handle);
 /* Times every shape on steps_per_shape steps of this ensemble (HIP events

@@ -116,6 +116,8 @@ def hip_lib() -> C.CDLL:
         lib.nemo_ens_set_shape.argtypes = [vp, C.c_int]
         lib.nemo_ens_shape_columns.argtypes = [vp, C.c_int]
         lib.nemo_ens_shape_columns.restype = C.c_int
+        lib.nemo_ens_shape_prefetch.argtypes = [vp, C.c_int]
+        lib.nemo_ens_shape_prefetch.restype = C.c_int
         lib.nemo_ens_shape.argtypes = [vp]
         lib.nemo_ens_shape.restype = C.c_int
         lib.nemo_ens_autotune.argtypes = [vp, C.c_int, _dp]

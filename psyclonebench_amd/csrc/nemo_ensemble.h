@@ -1,7 +1,7 @@
 // Batched ensemble of independent NEMOLite2D domains (nemo_ensemble.hip):
 // every domain's arrays live in one arena per field, each domain at its own
-// element offset with its own pitch and internal bounds; one persistent
-// launch per step walks a tile -> domain work list.
+// element offset with its own pitch and internal bounds; one launch per step
+// walks a tile -> domain work list.
 #pragma once
 
 #include "nemo_mixed.h"
@@ -15,38 +15,59 @@ namespace nemo {
 struct EnsDomain {
     unsigned off;
     int W, xs, xe, ys, ye;
+};
+
+// One tile of the work list: strip `strip` (of `cols` owned columns, 56 or
+// 61) by band `band` of domain `dom`, with that domain's arena offset, pitch
+// and internal bounds copied in, so that a wavefront reads one descriptor
+// (two independent scalar loads) and no domain table before it issues its
+// first row of loads. Tiles are listed domain by domain, bands outer, strips
+// inner.
+struct alignas(16) EnsTile {
+    unsigned off;
+    int W, xs, xe, ys, ye;
+    int strip, band, dom, cols;
     int pad[2];
 };
+static_assert(sizeof(EnsTile) == 48, "tile descriptor: 12 dwords");
 
-// One tile of the work list: domain, strip (61 columns), band (R rows).
-// Tiles are listed domain by domain, bands outer, strips inner.
-struct EnsTile {
-    int dom, strip, band, pad;
-};
-
-// Launch shape of the persistent step kernel.
+// Launch shape of the step kernel.
 struct EnsShape {
     int rows;              // rows per band (tile height): 4, 8, 16, 32 or 64
     int waves_per_block;   // wavefronts per workgroup: 1, 2 or 4
-    int blocks;            // workgroups in the launch (a multiple of 8)
-    int cols;              // owned columns per wavefront (tile width): 61 or 56
+    int blocks;            // persistent: workgroups in the launch (a multiple
+                           // of 8); 0: one one-wavefront workgroup per tile
+    int cols;              // owned columns per wavefront (tile width): 61 or
+                           // 56, or 0: per domain, whichever of the two its
+                           // width runs faster with (one wavefront per tile
+                           // only)
+    int prefetch;          // rows of loads issued ahead of the row that needs
+                           // them, beyond the next: 0 or 1
 };
+
+// True when a kernel instantiation exists for the shape's launch form, rows,
+// columns and prefetch depth.
+bool ens_shape_supported(const EnsShape &shape);
 
 // FP64 step of every domain: state `in` -> `out` (arena base pointers).
 // tide: the bc_ssh elevation of each domain for this step.
 void launch_ens_f64(hipStream_t s, const EnsShape &shape, const StepIn<double> &in,
                     const StepScalars<double> &p, const StepOut<double> &out,
-                    const EnsDomain *dom, const EnsTile *tiles, int ntiles,
-                    const double *tide);
+                    const EnsTile *tiles, int ntiles, const double *tide);
 
 // Mixed-precision step of every domain (FP32 fields, compensated ssh).
 void launch_ens_mixed(hipStream_t s, const EnsShape &shape, const MixedIn &in,
                       const StepScalars<float> &p, const MixedOut &out,
-                      const EnsDomain *dom, const EnsTile *tiles, int ntiles,
-                      const double *tide);
+                      const EnsTile *tiles, int ntiles, const double *tide);
 
-// Resident wavefronts per CU of the step kernels (from their register use),
-// for sizing persistent launches.
-int ens_max_waves_per_cu(int precision);
+// Tile width of domain `b` for shapes of cols = 0: 61 where the domain needs
+// enough fewer 61-column strips than 56-column ones to make up for the
+// slower 61-column tile (a strip costs about 8% more, both precisions,
+// measured on the ensemble workload), else 56.
+int ens_domain_cols(const Bounds &b);
+
+// Resident wavefronts per CU of the persistent step kernel of these rows,
+// columns and prefetch depth (from its register use), for sizing its launch.
+int ens_max_waves_per_cu(int precision, int rows, int cols, int prefetch);
 
 }  // namespace nemo

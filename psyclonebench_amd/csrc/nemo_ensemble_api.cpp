@@ -1,6 +1,6 @@
 // C ABI of the batched ensemble (no counterpart in the reference): many
 // independent NEMOLite2D domains of uneven size, bathymetry and tide,
-// advanced together by one persistent launch per step (plus one Flather
+// advanced together by one launch per step (plus one Flather
 // launch over every domain's boundary segments).
 //
 // Memory: one arena per field holding every domain back to back, each at an
@@ -10,12 +10,15 @@
 // each domain); precision 32 runs the mixed form (FP32 fields, compensated
 // ssh, FP64 tide), bitwise the single-domain mixed form of each domain.
 //
-// Launch shapes (rows per band x columns per wavefront x wavefronts per
-// workgroup x resident wavefronts per CU) are enumerated at creation; nemo_ens_autotune times each
+// Launch shapes (rows per band x columns per wavefront x prefetch depth x
+// launch form: one wavefront per tile, or persistent with so many wavefronts
+// per workgroup and resident wavefronts per CU) are enumerated at creation;
+// nemo_ens_autotune times each
 // on the ensemble itself with HIP events and keeps the fastest, restoring the
 // state it advanced.
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <memory>
 #include <tuple>
@@ -31,9 +34,16 @@ namespace nemo {
 namespace {
 
 constexpr int kProg = 6;  // prognostic arenas per set: t (hi), t lo, u, v, su, sv
+// Arena k starts k * 4352 bytes (17 x 256) into its allocation. hipMalloc
+// hands out large allocations at the same large alignment, so without the
+// shift the ~21 arrays a tile reads at one element offset all start on the
+// same L2 channel and HBM bank; the shift spreads them (1-2% faster step on
+// the 256-domain ensemble, both precisions). A multiple of 256 keeps every
+// row's 64-byte alignment.
+constexpr size_t kArenaStagger = 4352;
 // tile shapes of the launch list: rows per band x owned columns per wavefront
 constexpr int kEnsRows[] = {4, 8, 16, 32, 64};
-constexpr int kEnsCols[] = {61, 56};
+constexpr int kEnsCols[] = {61, 56, 0};  // 0: per domain, 56 or 61
 
 struct EnsContext {
     int prec = 64;
@@ -44,7 +54,6 @@ struct EnsContext {
     std::vector<double> amp, omega_tide;
     size_t total = 0;        // elements per arena
     long long interior = 0;  // interior T points over all domains
-    EnsDomain *ddom = nullptr;
     int *tm = nullptr;
     signed char *tm8 = nullptr;  // the mask in one byte a point (FP32 ensembles)
     // constant arenas by C-ABI index (A_HT, ..., A_DY_T) + Coriolis factors
@@ -63,6 +72,9 @@ struct EnsContext {
     std::map<std::tuple<int, int, int, int>, hipGraphExec_t> graphs;  // n, parity, shape, row0
     std::vector<hipEvent_t> ev;
     double last_ms = -1.0;
+    // arena allocations as hipMalloc returned them; arena k starts
+    // k * kArenaStagger bytes into its allocation
+    std::vector<void *> raw;
 
     size_t esize() const { return prec == 64 ? sizeof(double) : sizeof(float); }
 
@@ -140,13 +152,9 @@ struct EnsContext {
     {
         for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
-        for (void *p : cst)
-            if (p) (void)hipFree(p);
-        for (auto &set : prog)
-            for (void *p : set)
-                if (p) (void)hipFree(p);
+        for (void *p : raw) (void)hipFree(p);
         for (auto &kv : lists) (void)hipFree(kv.second.first);
-        void *more[] = {ddom, tm, tm8, fcor_u, fcor_v, d_tide};
+        void *more[] = {d_tide};
         for (void *p : more)
             if (p) (void)hipFree(p);
         if (h_tide) (void)hipHostFree(h_tide);
@@ -201,12 +209,15 @@ void put_ssh(EnsContext &C, int set, int d, const double *h, hipStream_t s)
     NEMO_HIP_CHECK(hipStreamSynchronize(s));
 }
 
-void *arena(const EnsContext &C, size_t esize, hipStream_t s)
+void *arena(EnsContext &C, size_t esize, hipStream_t s)
 {
+    const size_t shift = C.raw.size() * kArenaStagger;
     void *p = nullptr;
-    NEMO_HIP_CHECK(hipMalloc(&p, C.total * esize));
-    NEMO_HIP_CHECK(hipMemsetAsync(p, 0, C.total * esize, s));
-    return p;
+    NEMO_HIP_CHECK(hipMalloc(&p, C.total * esize + shift));
+    C.raw.push_back(p);
+    void *a = static_cast<char *>(p) + shift;
+    NEMO_HIP_CHECK(hipMemsetAsync(a, 0, C.total * esize, s));
+    return a;
 }
 
 void build_shapes(EnsContext &C)
@@ -216,40 +227,60 @@ void build_shapes(EnsContext &C)
     NEMO_HIP_CHECK(hipGetDevice(&dev));
     NEMO_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
     const int cus = prop.multiProcessorCount;
-    const int occ = ens_max_waves_per_cu(C.prec);  // resident waves per CU
-    if (occ < 1) NEMO_FATAL("ensemble step kernel does not fit on a CU");
-    const int levels[3] = {std::max(1, occ / 3), std::max(1, (2 * occ) / 3), occ};
     for (int cols : kEnsCols)
         for (int rows : kEnsRows)
-            for (int wpb : {1, 2, 4})
-                for (int lv : levels) {
-                    const int per_cu = std::max(1, lv / wpb);
-                    int blocks = cus * per_cu;
-                    blocks = (blocks + 7) / 8 * 8;
-                    C.shapes.push_back(EnsShape{rows, wpb, blocks, cols});
-                }
+            for (int pf : {0, 1}) {
+                // one wavefront per tile
+                const EnsShape grid{rows, 1, 0, cols, pf};
+                if (ens_shape_supported(grid)) C.shapes.push_back(grid);
+                // persistent, at a third, two thirds and all of the kernel's
+                // residency
+                if (!ens_shape_supported(EnsShape{rows, 1, 8, cols, pf})) continue;
+                const int occ = ens_max_waves_per_cu(C.prec, rows, cols, pf);
+                if (occ < 1) NEMO_FATAL("ensemble step kernel does not fit on a CU");
+                const int levels[3] = {std::max(1, occ / 3), std::max(1, (2 * occ) / 3), occ};
+                for (int wpb : {1, 2, 4})
+                    for (int lv : levels) {
+                        const int per_cu = std::max(1, lv / wpb);
+                        int blocks = cus * per_cu;
+                        blocks = (blocks + 7) / 8 * 8;
+                        C.shapes.push_back(EnsShape{rows, wpb, blocks, cols, pf});
+                    }
+            }
 }
 
+// One tile list per (rows, columns) that some launch shape uses.
 void build_lists(EnsContext &C, hipStream_t s)
 {
     std::vector<std::vector<EnsTile>> host;  // alive until the copies are done
-    for (int cols : kEnsCols)
-        for (int rows : kEnsRows) {
-            std::vector<EnsTile> L;
-            for (int d = 0; d < C.ndom; d++) {
-                const Geometry &G = C.geo[d];
-                const Bounds b{G.xs, G.xe, G.ys, G.ye};
-                const int ns = wave::strips(b, cols), nb = wave::bands(b, rows);
-                for (int band = 0; band < nb; band++)
-                    for (int strip = 0; strip < ns; strip++)
-                        L.push_back(EnsTile{d, strip, band, 0});
-            }
-            EnsTile *dl = dev_alloc<EnsTile>(L.size());
-            NEMO_HIP_CHECK(hipMemcpyAsync(dl, L.data(), L.size() * sizeof(EnsTile),
-                                          hipMemcpyHostToDevice, s));
-            C.lists[{rows, cols}] = {dl, (int)L.size()};
-            host.push_back(std::move(L));
+    for (const EnsShape &sh : C.shapes) {
+        const int rows = sh.rows, cols = sh.cols;
+        if (C.lists.count({rows, cols})) continue;
+        std::vector<EnsTile> L;
+        std::vector<int> ow(C.ndom), ns(C.ndom), nb(C.ndom);
+        for (int d = 0; d < C.ndom; d++) {
+            const EnsDomain &e = C.hdom[d];
+            const Bounds b{e.xs, e.xe, e.ys, e.ye};
+            ow[d] = cols ? cols : ens_domain_cols(b);
+            ns[d] = wave::strips(b, ow[d]);
+            nb[d] = wave::bands(b, rows);
         }
+        auto push = [&](int d, int band, int strip) {
+            const EnsDomain &e = C.hdom[d];
+            L.push_back(EnsTile{e.off, e.W, e.xs, e.xe, e.ys, e.ye, strip, band, d, ow[d],
+                                {0, 0}});
+        };
+        for (int d = 0; d < C.ndom; d++)
+            for (int band = 0; band < nb[d]; band++)
+                for (int strip = 0; strip < ns[d]; strip++) push(d, band, strip);
+        if (L.size() > (size_t)std::numeric_limits<int>::max())
+            NEMO_FATAL("ensemble: %zu tiles of %d rows exceed one launch", L.size(), rows);
+        EnsTile *dl = dev_alloc<EnsTile>(L.size());
+        NEMO_HIP_CHECK(hipMemcpyAsync(dl, L.data(), L.size() * sizeof(EnsTile),
+                                      hipMemcpyHostToDevice, s));
+        C.lists[{rows, cols}] = {dl, (int)L.size()};
+        host.push_back(std::move(L));
+    }
     NEMO_HIP_CHECK(hipStreamSynchronize(s));
 }
 
@@ -316,11 +347,11 @@ void launch_pair(EnsContext &C, int par, int row, hipStream_t s, hipEvent_t e0 =
     if (e0) NEMO_HIP_CHECK(hipEventRecord(e0, s));
     if (C.prec == 64) {
         const StepScalars<double> p{C.P.rdt, C.P.visc, C.P.g, 1.0 + C.P.cbfr * C.P.rdt};
-        launch_ens_f64(s, sh, C.in64(par), p, C.out64(q), C.ddom, L.first, L.second, tide);
+        launch_ens_f64(s, sh, C.in64(par), p, C.out64(q), L.first, L.second, tide);
     } else {
         const StepScalars<float> p{(float)C.P.rdt, (float)C.P.visc, (float)C.P.g,
                                    (float)(1.0 + C.P.cbfr * C.P.rdt)};
-        launch_ens_mixed(s, sh, C.in32(par), p, C.out32(q), C.ddom, L.first, L.second, tide);
+        launch_ens_mixed(s, sh, C.in32(par), p, C.out32(q), L.first, L.second, tide);
     }
     if (e1) NEMO_HIP_CHECK(hipEventRecord(e1, s));
     const int prev = C.paired ? par : q;  // the previous step's velocities
@@ -500,17 +531,18 @@ void *nemo_ens_create(int ndom, void *const *arrays, const int *geom,
         }
         NEMO_HIP_CHECK(hipStreamSynchronize(s));  // fu / fv / tm8 go out of scope
     }
-    C->ddom = dev_alloc<EnsDomain>((size_t)ndom);
-    NEMO_HIP_CHECK(hipMemcpyAsync(C->ddom, C->hdom.data(), ndom * sizeof(EnsDomain),
-                                  hipMemcpyHostToDevice, s));
     build_flather_lists(*C, arrays, s);
-    build_lists(*C, s);
     build_shapes(*C);
-    // default: 32 rows x 61 columns, 1 wavefront per workgroup, full residency
-    for (size_t k = 0; k < C->shapes.size(); k++)
-        if (C->shapes[k].rows == 32 && C->shapes[k].cols == 61 &&
-            C->shapes[k].waves_per_block == 1)
+    build_lists(*C, s);
+    // default: one wavefront per tile, 4 rows x 56 columns, loads two rows
+    // ahead in FP64 and one in FP32 (the fastest shapes on a 256-domain
+    // ensemble of sides 64..1024 on MI355X)
+    const int pf = precision == 64 ? 1 : 0;
+    for (size_t k = 0; k < C->shapes.size(); k++) {
+        const EnsShape &sh = C->shapes[k];
+        if (sh.blocks == 0 && sh.rows == 4 && sh.cols == 56 && sh.prefetch == pf)
             C->shape = (int)k;
+    }
     NEMO_HIP_CHECK(hipStreamSynchronize(s));
     return C.release();
 }
@@ -543,6 +575,14 @@ int nemo_ens_shape_columns(void *h, int shape)
     if (shape < 0 || shape >= (int)C.shapes.size())
         NEMO_FATAL("unknown ensemble shape %d", shape);
     return C.shapes[shape].cols;
+}
+
+int nemo_ens_shape_prefetch(void *h, int shape)
+{
+    EnsContext &C = ctx(h);
+    if (shape < 0 || shape >= (int)C.shapes.size())
+        NEMO_FATAL("unknown ensemble shape %d", shape);
+    return C.shapes[shape].prefetch;
 }
 
 void nemo_ens_steps(void *h, int nsteps, int istep) { run(ctx(h), nsteps, istep, false); }

@@ -84,15 +84,17 @@ class EnsembleStepper:
     def interior_points(self) -> int:
         return int(hip_lib().nemo_ens_interior_points(self.handle))
 
-    def shapes(self) -> list[tuple[int, int, int, int]]:
-        """(rows per band, wavefronts per workgroup, workgroups, columns per
-        wavefront) per shape."""
+    def shapes(self) -> list[tuple[int, int, int, int, int]]:
+        """(rows per band, wavefronts per workgroup, workgroups (0: one per
+        tile, else a persistent launch), columns per wavefront, prefetch
+        depth) per shape."""
         lib = hip_lib()
         out = []
         for k in range(lib.nemo_ens_shape_count(self.handle)):
             r, w, b = C.c_int(), C.c_int(), C.c_int()
             lib.nemo_ens_shape_info(self.handle, k, C.byref(r), C.byref(w), C.byref(b))
-            out.append((r.value, w.value, b.value, lib.nemo_ens_shape_columns(self.handle, k)))
+            out.append((r.value, w.value, b.value, lib.nemo_ens_shape_columns(self.handle, k),
+                        lib.nemo_ens_shape_prefetch(self.handle, k)))
         return out
 
     @property

@@ -10,7 +10,9 @@ FETCH_SIZE / WRITE_SIZE counters. Prints the dispatch geometry and the
 algorithmic byte counts the profiles are compared with. `--closure-train`
 runs training steps of the learned closure (forward, loss, backward, Adam)
 on synthetic input / target pairs over the 1024^2 coarse grid of the same
-workload, uploaded from the host each step.
+workload, uploaded from the host each step. `--ensemble` runs steps of the
+256-domain ensemble (seed 0, sides log-uniform in [64, 1024]) as plain
+launches, per precision and one-wavefront-per-tile shape.
 """
 from __future__ import annotations
 
@@ -38,6 +40,15 @@ def main() -> None:
     p.add_argument("--mixed-cols", type=str, default="61",
                    help="output columns per wavefront of the mixed kernel (61, 48), "
                         "comma separated")
+    p.add_argument("--mixed-prefetch", type=int, default=0,
+                   help="rows of loads the mixed kernel issues ahead (0 or 1)")
+    p.add_argument("--ensemble", type=int, default=0,
+                   help="ensemble steps (plain launches) per precision and shape")
+    p.add_argument("--ens-domains", type=int, default=256)
+    p.add_argument("--ens-precision", type=str, default="64,32")
+    p.add_argument("--ens-shapes", type=str, default="4x56p1",
+                   help="one-wavefront-per-tile ensemble shapes ROWSxCOLSpPREFETCH "
+                        "(COLS 0: per domain), comma separated")
     p.add_argument("--closure-train", type=int, default=0,
                    help="closure training steps on synthetic coarse-grid pairs")
     a = p.parse_args()
@@ -76,11 +87,31 @@ def main() -> None:
         out["mixed_kernel_ms"] = {}
         for cols in [int(c) for c in a.mixed_cols.split(",")]:
             for rows in [int(r) for r in a.mixed_rows.split(",")]:
-                st = MixedStepper(Model(cfg), rows=rows, cols=cols)
+                st = MixedStepper(Model(cfg), rows=rows, cols=cols, prefetch=a.mixed_prefetch)
                 st.run(1)
                 out["mixed_kernel_ms"][f"{rows}x{cols}"] = st.timed_run(a.mixed)
                 st.release()
         out["mixed_alg_bytes_per_launch"] = 113 * (cfg.jpiglo - 2) * (cfg.jpjglo - 2)
+    if a.ensemble:
+        from .ensemble import EnsembleStepper, ensemble_configs
+        models = [Model(c) for c in ensemble_configs(a.ens_domains, seed=0)]
+        out["ensemble_kernel_ms"] = {}
+        for prec in [int(q) for q in a.ens_precision.split(",")]:
+            for spec in a.ens_shapes.split(","):
+                rc, pf = spec.split("p")
+                rows, cols = (int(x) for x in rc.split("x"))
+                ens = EnsembleStepper(models, precision=prec)
+                k = [i for i, sh in enumerate(ens.shapes())
+                     if sh[2] == 0 and (sh[0], sh[3], sh[4]) == (rows, cols, int(pf))]
+                if not k:
+                    raise SystemExit(f"no one-wavefront-per-tile ensemble shape {spec}")
+                ens.set_shape(k[0])
+                ens.run(1)
+                out["ensemble_kernel_ms"][f"f{prec} {spec}"] = ens.timed_run(a.ensemble)
+                pts = ens.interior_points
+                ens.release()
+        out["ensemble_interior_points"] = pts
+        out["ensemble_alg_bytes_per_launch"] = {"f64": B_ALG_F64 * pts, "f32": 113 * pts}
     if a.closure_train:
         import numpy as np
 

@@ -62,13 +62,17 @@ def test_ensemble_f64_matches_oracle_per_domain():
 
 @pytest.mark.parametrize("precision", [64, 32])
 def test_ensemble_every_shape_bitwise_equal(precision):
-    """Every launch shape (rows x columns x wavefronts per workgroup x
-    residency) gives the same bits, in both precisions."""
+    """Every launch shape (rows x columns x prefetch depth x launch form:
+    one wavefront per tile, or persistent with so many wavefronts per
+    workgroup and residency) gives the same bits, in both precisions."""
     cfgs = small_ensemble(ndom=5, seed=7)
     models = [Model(c) for c in cfgs]
     probe = EnsembleStepper(models, precision=precision)
     nshape = len(probe.shapes())
-    assert {s[3] for s in probe.shapes()} == {61, 56}
+    shapes = probe.shapes()
+    assert {s[3] for s in shapes} == {61, 56, 0}
+    assert {s[4] for s in shapes} == {0, 1}
+    assert any(s[2] == 0 for s in shapes) and any(s[2] > 0 for s in shapes)
     probe.release()
     ref = None
     for k in range(nshape):
