@@ -59,12 +59,15 @@ int nemo_setup_tmask(int mode, int M, int N, int W, int H, int32_t *tmask)
     // internal region (1-based 2..M-1 x 2..N-1) is wet
     for (int jj = 1; jj <= N - 2; jj++)
         for (int ji = 1; ji <= M - 2; ji++) tmask[at(W, ji, jj)] = 1;
-    // east column (all rows), then south row (all columns): the south rule
-    // is applied last in model_mod.f90, so it owns the corners.
+    // east column (all rows, model_mod.f90:191-199), then the south row (all
+    // columns, :215-224): -1 where the south edge is open, else 0. The south
+    // rule comes last, so it owns both southern corners: the south-east
+    // corner of an east-open mask is 0.
     if (mode == NEMO_MASK_EAST_OPEN || mode == NEMO_MASK_SOUTH_EAST)
         for (int jj = 0; jj < N; jj++) tmask[at(W, M - 1, jj)] = -1;
-    if (mode == NEMO_MASK_SOUTH_OPEN || mode == NEMO_MASK_SOUTH_EAST)
-        for (int ji = 0; ji < M; ji++) tmask[at(W, ji, 0)] = -1;
+    const int south = (mode == NEMO_MASK_SOUTH_OPEN || mode == NEMO_MASK_SOUTH_EAST) ? -1 : 0;
+    if (mode != NEMO_MASK_WEST_OPEN)
+        for (int ji = 0; ji < M; ji++) tmask[at(W, ji, 0)] = south;
     if (mode == NEMO_MASK_WEST_OPEN)
         for (int jj = 0; jj < N; jj++) tmask[at(W, 0, jj)] = -1;
     return 0;

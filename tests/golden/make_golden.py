@@ -4,7 +4,8 @@ Run in a container that has the reference tree (the fixtures are committed,
 so nothing else needs it):
 
     make -C oracle            # builds oracle/_ref/ from the reference sources
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py                     # every fixture
+    python tests/golden/make_golden.py --only ref_s32_east # one step case
 
 Every expected output comes from the reference C++ PSy layer
 (manual_versions/psykal_cpp/time_step.cpp, built by oracle/Makefile into
@@ -235,9 +236,22 @@ def make_kernels() -> dict:
 
 
 def main() -> None:
+    import argparse
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--only", action="append", default=[],
+                    help="rewrite only these step cases (and their manifest entries)")
+    a = ap.parse_args()
     if not oracle.ref_available():
         raise SystemExit("oracle/_ref not built: run `make -C oracle` with the "
                          "reference tree present")
+    if a.only:
+        manifest = json.loads((HERE / "manifest.json").read_text())
+        for c in STEP_CASES:
+            if c["name"] in a.only:
+                print("case", c["name"], flush=True)
+                manifest["steps"][c["name"]] = run_case(c)
+        (HERE / "manifest.json").write_text(json.dumps(manifest, indent=1) + "\n")
+        return
     gxx = subprocess.run(["g++", "--version"], capture_output=True, text=True).stdout
     manifest = dict(
         generator="tests/golden/make_golden.py",

@@ -26,6 +26,41 @@ def test_grid_dims_and_region():
     assert g.bounds == (1, 128, 1, 128)
 
 
+def reference_tmask(M: int, N: int, W: int, H: int, mode: int) -> np.ndarray:
+    """setup_tpoints_mask (model_mod.f90:136-246) for a single domain (its
+    global extent: west, north, east and south edges all global), one
+    statement at a time in the reference's order, [row, column] 0-based,
+    zero in the allocation padding. The west-open mode (no reference
+    counterpart) sets the west column to -1 in every row after the south rule."""
+    t = np.zeros((H, W), np.int32)
+    t[1:N - 1, 1:M - 1] = 1                                      # :146-147
+    t[:N, 0] = 0                                                 # :152-154 west solid
+    t[N - 1, :M] = 0                                             # :171-173 north solid
+    t[:N, M - 1] = -1 if mode in (MASK_EAST_OPEN, MASK_SOUTH_EAST) else 0   # :191-199
+    t[0, :M] = -1 if mode in (MASK_SOUTH_OPEN, MASK_SOUTH_EAST) else 0      # :215-224
+    if mode == MASK_WEST_OPEN:
+        t[:N, 0] = -1
+    return t
+
+
+@pytest.mark.parametrize("mode", [MASK_CLOSED, MASK_SOUTH_OPEN, MASK_EAST_OPEN,
+                                  MASK_SOUTH_EAST, MASK_WEST_OPEN])
+@pytest.mark.parametrize("n", [(10, 8), (34, 34), (7, 19)])
+def test_tmask_equals_reference_rules(mode, n):
+    g = Model(NemoConfig(jpiglo=n[0], jpjglo=n[1], jphgr_msh=mode)).grid
+    assert np.array_equal(g.tmask, reference_tmask(g.M, g.N, g.W, g.H, mode))
+
+
+def test_east_open_south_east_corner_is_closed():
+    """The south rule runs after the east rule (model_mod.f90:215-224 after
+    :191-199), so an east-open mask has 0, not -1, at its south-east
+    corner; south-east-open has -1 there."""
+    for mode, corner in ((MASK_EAST_OPEN, 0), (MASK_SOUTH_EAST, -1)):
+        g = Model(NemoConfig(jpiglo=12, jpjglo=9, jphgr_msh=mode)).grid
+        assert g.tmask[0, g.M - 1] == corner
+        assert g.tmask[g.N - 1, g.M - 1] == -1       # north-east: east rule wins
+
+
 @pytest.mark.parametrize("mode", [MASK_CLOSED, MASK_SOUTH_OPEN, MASK_EAST_OPEN,
                                   MASK_SOUTH_EAST, MASK_WEST_OPEN])
 def test_tmask_modes(mode):
@@ -56,15 +91,19 @@ def test_tmask_modes(mode):
 
 
 def test_golden_masks_match_setup():
-    """Every multi-step fixture stored its tmask/depths; the set-up library
-    must reproduce them (the fixtures were written from its output)."""
-    from helpers import model_for_case  # noqa: F401
+    """Every multi-step fixture stored the tmask and depths it ran the
+    reference with. Each stored tmask must follow the reference's mask
+    rules (reference_tmask, restated from model_mod.f90), and the set-up
+    library must still reproduce the stored tmask and depths (the fixtures
+    were written from its output, so that part is a regression check)."""
     for name, c in manifest()["steps"].items():
         cfg = reference_config(c["n"], 1, mask=c["mask"], cbfr=c["cbfr"])
         if c.get("bathy"):
             cfg.bathymetry = Bathymetry(**c["bathy"])
         m = Model(cfg)
+        g = m.grid
         fx = load_npz(f"steps_{name}.npz")
+        assert np.array_equal(fx["in_tmask"], reference_tmask(g.M, g.N, g.W, g.H, c["mask"])), name
         for k in ("tmask", "ht", "hu", "hv"):
             assert np.array_equal(m.array(k), fx[f"in_{k}"]), (name, k)
 
