@@ -5,11 +5,22 @@
 ! time_step_mod.f90:11-58: c_invoke_time_step with 25 host arrays, 25
 ! device-handle slots and the scalars; c_read_from_device), so this program
 ! shows a Fortran host driving the HIP step through exactly that interface.
-! It reads the reference's namelist (&namctl with the defaults of
+!
+! Module hip_fields runs the field protocol of that PSy layer in this
+! program's own types (dl_esm_inf's field_mod and grid_mod are not in this
+! repository): each field carries its data, a device-handle slot, a
+! data_on_device flag and a read_from_device_c procedure pointer;
+! invoke_time_step passes the data and the slots to c_invoke_time_step, and
+! after its first call marks the fields device-resident and installs
+! c_read_from_device as their read-back (time_step_mod.f90:76-192);
+! field_get_data reads a device-resident field back through that pointer
+! from its current slot, as dl_esm_inf's get_data does.
+!
+! The program reads the reference's namelist (&namctl with the defaults of
 ! common/gocean2d_io_mod.f90:58-69), sets the model up through
-! libnemo_setup.so (dl_esm_inf and model_mod are not in this repository),
-! steps nit000..nitend and logs the checksums of the reference driver
-! (psykal_c_opencl/nemolite2d.f90:116-120).
+! libnemo_setup.so, steps nit000..nitend, logs the ssh checksum every
+! irecord steps through field_get_data, and at the end the checksums of the
+! reference driver (psykal_c_opencl/nemolite2d.f90:116-120).
 module nemo_hip_abi
   use iso_c_binding
   implicit none
@@ -89,9 +100,123 @@ module nemo_hip_abi
   end interface
 end module nemo_hip_abi
 
+module hip_fields
+  use iso_c_binding
+  use nemo_hip_abi
+  implicit none
+
+  abstract interface
+     subroutine read_from_device_iface(from, to, startx, starty, nx, ny, blocking) bind(C)
+       import :: c_ptr, c_int, c_bool
+       type(c_ptr), value :: from, to
+       integer(c_int), value :: startx, starty, nx, ny
+       logical(c_bool), value :: blocking
+     end subroutine read_from_device_iface
+  end interface
+
+  ! One field: host data (width x height, first index fastest), the device
+  ! handle the library keeps in its slot, and the read-back hook.
+  type :: hip_field
+     real(c_double), allocatable :: data(:,:)
+     type(c_ptr) :: device_ptr = c_null_ptr
+     logical :: data_on_device = .false.
+     procedure(read_from_device_iface), nopass, pointer :: read_from_device_c => null()
+  end type hip_field
+
+  ! The grid's arrays and their device slots; internal bounds 1-based.
+  type :: hip_grid
+     integer :: xstart, xstop, ystart, ystop
+     integer(c_int), allocatable :: tmask(:,:)
+     real(c_double), allocatable, dimension(:,:) :: area_t, area_u, area_v, dx_u, dx_v, &
+          dx_t, dy_u, dy_v, dy_t, gphiu, gphiv
+     type(c_ptr) :: tmask_device = c_null_ptr, area_t_device = c_null_ptr, &
+          area_u_device = c_null_ptr, area_v_device = c_null_ptr, dx_u_device = c_null_ptr, &
+          dx_v_device = c_null_ptr, dx_t_device = c_null_ptr, dy_u_device = c_null_ptr, &
+          dy_v_device = c_null_ptr, dy_t_device = c_null_ptr, gphiu_device = c_null_ptr, &
+          gphiv_device = c_null_ptr
+  end type hip_grid
+
+contains
+
+  subroutine field_alloc(f, w, h)
+    type(hip_field), intent(inout) :: f
+    integer, intent(in) :: w, h
+    allocate(f%data(w, h))
+    f%data = 0.0_c_double
+  end subroutine field_alloc
+
+  ! Host copy of a field: read back from the device when the data lives
+  ! there and a read-back is installed, else the host data as it is.
+  subroutine field_get_data(f)
+    type(hip_field), intent(inout), target :: f
+    if (f%data_on_device .and. associated(f%read_from_device_c)) then
+       call f%read_from_device_c(f%device_ptr, c_loc(f%data), 0_c_int, 0_c_int, &
+            int(size(f%data, 1), c_int), int(size(f%data, 2), c_int), .true._c_bool)
+    end if
+  end subroutine field_get_data
+
+  ! One step through the C ABI with the device-resident layer's protocol:
+  ! the host data and the device slots of every field and grid array, the
+  ! 0-based internal bounds, the row pitch and the total size.
+  subroutine invoke_time_step(istp, grid, ssha_t, ssha_u, ssha_v, sshn_t, sshn_u, sshn_v, &
+       hu, hv, ht, ua, va, un, vn, rdt, cbfr, visc, omega, d2r, g)
+    integer, intent(in) :: istp
+    type(hip_grid), intent(inout) :: grid
+    type(hip_field), intent(inout) :: ssha_t, ssha_u, ssha_v, sshn_t, sshn_u, sshn_v, hu, &
+         hv, ht, ua, va, un, vn
+    real(c_double), intent(in) :: rdt, cbfr, visc, omega, d2r, g
+    logical, save :: first_time = .true.
+
+    call c_invoke_time_step(ssha_t%data, sshn_t%data, sshn_u%data, sshn_v%data, hu%data, &
+         hv%data, un%data, vn%data, ua%data, ht%data, ssha_u%data, va%data, ssha_v%data, &
+         grid%tmask, grid%area_t, grid%area_u, grid%area_v, grid%dx_u, grid%dx_v, &
+         grid%dx_t, grid%dy_u, grid%dy_v, grid%dy_t, grid%gphiu, grid%gphiv, &
+         ssha_t%device_ptr, sshn_t%device_ptr, sshn_u%device_ptr, sshn_v%device_ptr, &
+         hu%device_ptr, hv%device_ptr, un%device_ptr, vn%device_ptr, ua%device_ptr, &
+         ht%device_ptr, ssha_u%device_ptr, va%device_ptr, ssha_v%device_ptr, &
+         grid%tmask_device, grid%area_t_device, grid%area_u_device, grid%area_v_device, &
+         grid%dx_u_device, grid%dx_v_device, grid%dx_t_device, grid%dy_u_device, &
+         grid%dy_v_device, grid%dy_t_device, grid%gphiu_device, grid%gphiv_device, &
+         int(istp, c_int), int(grid%xstart - 1, c_int), int(grid%xstop - 1, c_int), &
+         int(grid%ystart - 1, c_int), int(grid%ystop - 1, c_int), &
+         int(size(ssha_t%data, 1), c_int), int(size(ssha_t%data), c_int), &
+         rdt, cbfr, visc, omega, d2r, g)
+
+    if (first_time) then
+       first_time = .false.
+       ssha_t%data_on_device = .true.
+       sshn_t%data_on_device = .true.
+       sshn_u%data_on_device = .true.
+       sshn_v%data_on_device = .true.
+       hu%data_on_device = .true.
+       hv%data_on_device = .true.
+       un%data_on_device = .true.
+       vn%data_on_device = .true.
+       ua%data_on_device = .true.
+       ht%data_on_device = .true.
+       ssha_u%data_on_device = .true.
+       va%data_on_device = .true.
+       ssha_v%data_on_device = .true.
+       ! the fields the step writes get a read-back; the depths never change
+       ssha_t%read_from_device_c => c_read_from_device
+       sshn_t%read_from_device_c => c_read_from_device
+       sshn_u%read_from_device_c => c_read_from_device
+       sshn_v%read_from_device_c => c_read_from_device
+       un%read_from_device_c => c_read_from_device
+       vn%read_from_device_c => c_read_from_device
+       ua%read_from_device_c => c_read_from_device
+       ssha_u%read_from_device_c => c_read_from_device
+       va%read_from_device_c => c_read_from_device
+       ssha_v%read_from_device_c => c_read_from_device
+    end if
+  end subroutine invoke_time_step
+
+end module hip_fields
+
 program nemolite2d_hip
   use iso_c_binding
   use nemo_hip_abi
+  use hip_fields
   implicit none
   ! common/physical_params_mod.f90:9-15
   real(c_double), parameter :: pi = 3.1415926535897932_c_double
@@ -101,11 +226,9 @@ program nemolite2d_hip
   real(c_double) :: dx, dy, dep_const, rdt, cbfr, visc
   namelist /namctl/ jpiglo, jpjglo, jphgr_msh, dx, dy, dep_const, nit000, nitend, &
        irecord, rdt, cbfr, visc
-  real(c_double), allocatable, target, dimension(:,:) :: ssha_t, sshn_t, sshn_u, sshn_v, &
-       hu, hv, un, vn, ua, ht, ssha_u, va, ssha_v, area_t, area_u, area_v, dx_u, dx_v, &
-       dx_t, dy_u, dy_v, dy_t, gphiu, gphiv
-  integer(c_int), allocatable, target :: tmask(:,:)
-  type(c_ptr) :: d(25)
+  type(hip_field), target :: ssha_t, ssha_u, ssha_v, sshn_t, sshn_u, sshn_v, hu, hv, ht, &
+       ua, va, un, vn
+  type(hip_grid) :: grid
   integer(c_int) :: w, h
   integer :: istp, ios
 
@@ -119,37 +242,48 @@ program nemolite2d_hip
   close(99)
 
   call nemo_grid_dims(jpiglo, jpjglo, 64, w, h)
-  allocate(ssha_t(w,h), sshn_t(w,h), sshn_u(w,h), sshn_v(w,h), hu(w,h), hv(w,h), un(w,h), &
-       vn(w,h), ua(w,h), ht(w,h), ssha_u(w,h), va(w,h), ssha_v(w,h), area_t(w,h), &
-       area_u(w,h), area_v(w,h), dx_u(w,h), dx_v(w,h), dx_t(w,h), dy_u(w,h), dy_v(w,h), &
-       dy_t(w,h), gphiu(w,h), gphiv(w,h), tmask(w,h))
-  ssha_t = 0; sshn_t = 0; sshn_u = 0; sshn_v = 0; un = 0; vn = 0; ua = 0; va = 0
-  ssha_u = 0; ssha_v = 0
-  if (nemo_setup_tmask(jphgr_msh, jpiglo, jpjglo, w, h, tmask) /= 0) stop 'jphgr_msh'
-  call nemo_setup_metrics(w, h, dx, dy, 50.0_c_double, area_t, area_u, area_v, dx_u, dx_v, &
-       dx_t, dy_u, dy_v, dy_t, gphiu, gphiv)
-  call nemo_setup_depth_uniform(w, h, dep_const, ht, hu, hv)
-  call nemo_initialise(jpiglo, jpjglo, w, h, area_t, area_u, area_v, sshn_t, sshn_u, sshn_v, &
-       un, vn)
-  write(*, "('Simulation domain = (',I4,':',I4,',',I4,':',I4,')')") 2, jpiglo - 1, 2, jpjglo - 1
+  call field_alloc(ssha_t, w, h); call field_alloc(ssha_u, w, h)
+  call field_alloc(ssha_v, w, h); call field_alloc(sshn_t, w, h)
+  call field_alloc(sshn_u, w, h); call field_alloc(sshn_v, w, h)
+  call field_alloc(hu, w, h); call field_alloc(hv, w, h); call field_alloc(ht, w, h)
+  call field_alloc(ua, w, h); call field_alloc(va, w, h)
+  call field_alloc(un, w, h); call field_alloc(vn, w, h)
+  allocate(grid%tmask(w,h), grid%area_t(w,h), grid%area_u(w,h), grid%area_v(w,h), &
+       grid%dx_u(w,h), grid%dx_v(w,h), grid%dx_t(w,h), grid%dy_u(w,h), grid%dy_v(w,h), &
+       grid%dy_t(w,h), grid%gphiu(w,h), grid%gphiv(w,h))
+  grid%xstart = 2; grid%xstop = jpiglo - 1; grid%ystart = 2; grid%ystop = jpjglo - 1
+  if (nemo_setup_tmask(jphgr_msh, jpiglo, jpjglo, w, h, grid%tmask) /= 0) stop 'jphgr_msh'
+  call nemo_setup_metrics(w, h, dx, dy, 50.0_c_double, grid%area_t, grid%area_u, &
+       grid%area_v, grid%dx_u, grid%dx_v, grid%dx_t, grid%dy_u, grid%dy_v, grid%dy_t, &
+       grid%gphiu, grid%gphiv)
+  call nemo_setup_depth_uniform(w, h, dep_const, ht%data, hu%data, hv%data)
+  call nemo_initialise(jpiglo, jpjglo, w, h, grid%area_t, grid%area_u, grid%area_v, &
+       sshn_t%data, sshn_u%data, sshn_v%data, un%data, vn%data)
+  write(*, "('Simulation domain = (',I4,':',I4,',',I4,':',I4,')')") grid%xstart, &
+       grid%xstop, grid%ystart, grid%ystop
 
-  d = c_null_ptr
   do istp = nit000, nitend
-     call c_invoke_time_step(ssha_t, sshn_t, sshn_u, sshn_v, hu, hv, un, vn, ua, ht, ssha_u, &
-          va, ssha_v, tmask, area_t, area_u, area_v, dx_u, dx_v, dx_t, dy_u, dy_v, dy_t, &
-          gphiu, gphiv, d(1), d(2), d(3), d(4), d(5), d(6), d(7), d(8), d(9), d(10), d(11), &
-          d(12), d(13), d(14), d(15), d(16), d(17), d(18), d(19), d(20), d(21), d(22), &
-          d(23), d(24), d(25), istp, 1, jpiglo - 2, 1, jpjglo - 2, w, w * h, rdt, cbfr, &
-          visc, omega, d2r, g)
+     call invoke_time_step(istp, grid, ssha_t, ssha_u, ssha_v, sshn_t, sshn_u, sshn_v, hu, &
+          hv, ht, ua, va, un, vn, rdt, cbfr, visc, omega, d2r, g)
+     if (mod(istp, irecord) == 0) then
+        call field_get_data(sshn_t)
+        write(*, "('step ',I6,' sshn checksum = ', E26.18)") istp, &
+             nemo_field_checksum(sshn_t%data, w, 1, jpiglo - 2, 1, jpjglo - 2)
+     end if
   end do
 
-  ! slots 9 and 12: ua, va (each shares its buffer with un, vn after a step)
-  call c_read_from_device(d(9), c_loc(ua), 0, 0, w, h, .true._c_bool)
-  call c_read_from_device(d(12), c_loc(va), 0, 0, w, h, .true._c_bool)
-  call c_read_from_device(d(2), c_loc(sshn_t), 0, 0, w, h, .true._c_bool)
-  write(*, "('ua checksum = ', E16.8)") nemo_field_checksum(ua, w, 1, jpiglo - 2, 1, jpjglo - 2)
-  write(*, "('va checksum = ', E16.8)") nemo_field_checksum(va, w, 1, jpiglo - 2, 1, jpjglo - 2)
-  write(*, "('sshn checksum = ', E16.8)") &
-       nemo_field_checksum(sshn_t, w, 1, jpiglo - 2, 1, jpjglo - 2)
+  call field_get_data(ua)
+  call field_get_data(va)
+  call field_get_data(sshn_t)
+  write(*, "('ua checksum = ', E16.8)") nemo_field_checksum(ua%data, w, 1, jpiglo - 2, 1, &
+       jpjglo - 2)
+  write(*, "('va checksum = ', E16.8)") nemo_field_checksum(va%data, w, 1, jpiglo - 2, 1, &
+       jpjglo - 2)
+  write(*, "('sshn checksum = ', E16.8)") nemo_field_checksum(sshn_t%data, w, 1, jpiglo - 2, &
+       1, jpjglo - 2)
+  write(*, "('ua full checksum = ', E26.18)") nemo_field_checksum(ua%data, w, 1, jpiglo - 2, &
+       1, jpjglo - 2)
+  write(*, "('va full checksum = ', E26.18)") nemo_field_checksum(va%data, w, 1, jpiglo - 2, &
+       1, jpjglo - 2)
   write(*, "(A)") 'Simulation finished!!'
 end program nemolite2d_hip

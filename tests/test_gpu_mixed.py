@@ -130,7 +130,9 @@ def test_mixed_closed_basin_volume():
 @pytest.mark.slow
 def test_mixed_4096_error_report():
     """4096^2 seeded bathymetry, open west edge: mixed vs the FP64 fused
-    path (bitwise the oracle) after 250 / 500 / 1000 steps."""
+    path (bitwise the oracle) after 250 / 500 / 1000 steps; ssh within
+    1e-5 relmax after 1000 steps, un and vn within 1e-4 at every
+    checkpoint."""
     cfg = c2_config(4096, 1000, seed=0)
     f64 = TimeStepper(Model(cfg), mode=2)
     mp = MixedStepper(Model(cfg))
@@ -147,6 +149,9 @@ def test_mixed_4096_error_report():
     f64.release()
     mp.release()
     assert report[1000]["sshn_t"][0] <= SSH_RELMAX_TOL, report
+    for cp in report:
+        assert report[cp]["un"][0] <= VEL_RELMAX_TOL, report
+        assert report[cp]["vn"][0] <= VEL_RELMAX_TOL, report
 
 
 def test_mixed_random_mask_against_fp64():
