@@ -255,9 +255,10 @@ def closure_leg(a) -> dict:
 def ensemble_leg(a) -> dict:
     """The batched ensemble: --ens-domains domains (seed 0, sides
     log-uniform in [64, 1024], own bathymetry and tide), --ens-steps steps
-    per precision in one graph replay after an autotune of the persistent
-    launch shape; kernel roofline from HIP events; FP32 (mixed form) error
-    against FP64 after the same steps, worst domain per field."""
+    per precision in one graph replay after an autotune of the launch shape
+    (the default shape unless another is more than 1.5% faster); kernel
+    roofline from HIP events; FP32 (mixed form) error against FP64 after
+    the same steps, worst domain per field."""
     from psyclonebench_amd.ensemble import EnsembleStepper, ensemble_configs
     from psyclonebench_amd.model import Model
 
@@ -272,7 +273,13 @@ def ensemble_leg(a) -> dict:
     for prec, b_alg in ((64, B_ALG_F64), (32, B_ALG_MIXED)):
         ens = EnsembleStepper(models, precision=prec)
         pts = ens.interior_points
+        # the library's default shape stays unless another beats it by more
+        # than the 1.5% that 5-step timings vary by
+        default_shape = ens.shape
         best, tune_ms = ens.autotune(5)
+        if tune_ms[default_shape] <= 1.015 * tune_ms[best]:
+            best = default_shape
+            ens.set_shape(best)
         shape = ens.shapes()[best]
         ens.run(1 + a.warmup)
         ens.synchronize()
