@@ -325,13 +325,18 @@ def load_weights(path) -> tuple[np.ndarray, dict]:
 
 
 def evaluate(n: int, factor: int, bathy: list[int], steps: int, every: int, weights,
-             guard_rel: float | None = None, log_fn=log, trace_energy: bool = False) -> dict:
-    """guard_rel None: the energy-guard fraction the weights were trained with."""
+             guard_rel: float | None = None, log_fn=log, trace_energy: bool = False,
+             guard_max_out: float | None = None) -> dict:
+    """guard_rel None: the energy-guard fraction the weights were trained
+    with; guard_max_out None: their bound on each applied component."""
     params, meta = load_weights(weights)
-    norm = meta["norm"]
+    norm = dict(meta["norm"])
     if guard_rel is None:
         guard_rel = float(meta.get("guard_rel", 0.25))
-    out = {"seeds": {}, "every": every, "steps": steps, "guard_rel": guard_rel}
+    if guard_max_out is not None:
+        norm["guard_max_out"] = guard_max_out
+    out = {"seeds": {}, "every": every, "steps": steps, "guard_rel": guard_rel,
+           "guard_max_out": float(norm.get("guard_max_out", 0.0))}
     for b in bathy:
         fine, coarse = pair(n, factor, b, steps)
         plain = coarse.copy()
@@ -507,6 +512,9 @@ def main(argv=None) -> None:
     e.add_argument("--json", default=None)
     e.add_argument("--guard-rel", type=float, default=None,
                    help="energy-guard fraction (default: the one used in training)")
+    e.add_argument("--guard-max-out", type=float, default=None,
+                   help="bound on each applied component in normalised units (0: none; "
+                        "default: the one the weights were trained with)")
     e.add_argument("--trace-energy", action="store_true",
                    help="record the energy change of every correction (synchronises)")
     s = sub.add_parser("speed")
@@ -518,7 +526,8 @@ def main(argv=None) -> None:
         train(a)
     elif a.cmd == "eval":
         res = evaluate(a.n, a.factor, seeds(a.bathy), a.steps, a.every, a.weights,
-                       guard_rel=a.guard_rel, trace_energy=a.trace_energy)
+                       guard_rel=a.guard_rel, trace_energy=a.trace_energy,
+                       guard_max_out=a.guard_max_out)
         log(json.dumps({k: res[k] for k in ("ratio_final", "ratio_mean")}))
         if a.json:
             Path(a.json).write_text(json.dumps(res, indent=1))
