@@ -221,8 +221,9 @@ NEMO_API void nemo_mp_destroy(void *handle);
 
 /* ---- batched ensemble (no reference counterpart) -----------------------
  * ndom independent domains, each in its own dl_esm_inf layout, advanced
- * together: one persistent launch per step walks a tile -> domain work list,
- * then one launch runs every domain's Flather segments.
+ * together: one launch per step walks a tile -> domain work list (one
+ * wavefront per tile, or a persistent grid), then one launch runs every
+ * domain's Flather segments.
  *   arrays      ndom x 25 host pointers, each row in the c_invoke_time_step
  *               argument order (tmask as int*); read during creation only
  *   geom        ndom x 6 ints: xs, xe, ys, ye (0-based internal bounds),

@@ -13,7 +13,7 @@ each with an open west edge and the physics of the 128^2 case (dx = dy =
 1 km, dt = 20 s, cbfr 1.5e-3, visc 0.1).
 
 :class:`EnsembleStepper` advances a list of :class:`~.model.Model` together
-through ``nemo_ens_*`` (one persistent launch per step).
+through ``nemo_ens_*`` (one launch per step over a tile -> domain work list).
 """
 from __future__ import annotations
 
