@@ -521,6 +521,8 @@ def main(argv=None) -> None:
     s.add_argument("--n", type=int, default=4096)
     s.add_argument("--factor", type=int, default=4)
     s.add_argument("--weights", default=str(DEFAULT_WEIGHTS))
+    s.add_argument("--steps", type=int, default=200,
+                   help="coarse steps timed (the fine run covers the same model time)")
     a = p.parse_args(argv)
     if a.cmd == "train":
         train(a)
@@ -532,7 +534,7 @@ def main(argv=None) -> None:
         if a.json:
             Path(a.json).write_text(json.dumps(res, indent=1))
     else:
-        log(json.dumps(throughput(a.n, a.factor, a.weights)))
+        log(json.dumps(throughput(a.n, a.factor, a.weights, steps=a.steps)))
 
 
 if __name__ == "__main__":
