@@ -429,6 +429,13 @@ def throughput(n: int, factor: int, weights, steps: int = 200) -> dict:
     net.steps(sl, steps, guard=True)
     hip_lib().nemo_hip_synchronize()
     learned_ms = (time.perf_counter() - t0) * 1e3 / steps
+    # the same steps as host-issued launches (c_invoke_time_step + nemo_cl_correct)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sl.run(1)
+        net.correct(sl, guard=True)
+    hip_lib().nemo_hip_synchronize()
+    stepwise_ms = (time.perf_counter() - t0) * 1e3 / steps
     t0 = time.perf_counter()
     for _ in range(steps):
         sl.run(1)
@@ -444,6 +451,7 @@ def throughput(n: int, factor: int, weights, steps: int = 200) -> dict:
     sf.release()
     sl.release()
     return {"fine_ms_per_coarse_interval": fine_ms, "learned_ms_per_coarse_step": learned_ms,
+            "learned_stepwise_ms_per_coarse_step": stepwise_ms,
             "plain_ms_per_coarse_step": plain_ms, "forward_ms": float(np.median(fwd)),
             "speedup_vs_fine": fine_ms / learned_ms}
 
