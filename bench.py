@@ -239,7 +239,7 @@ def closure_leg(a) -> dict:
         "fine_ms_per_80s": sp["fine_ms_per_coarse_interval"],
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": achieved / 2500.0,
-                     "kernel": "k_conv_sh x3 + k_conv_out (4 layers, bf16 MFMA)",
+                     "kernel": "k_conv_in + k_conv_sh x2 + k_conv_out (4 layers, bf16 MFMA)",
                      "forward_ms": sp["forward_ms"], "flop_per_forward_alg": flops},
         "error_vs_fine": {"metric": "relative RMS over the interior vs the box-filtered "
                                     "fine state, mean over every 500 steps",

@@ -106,46 +106,26 @@ __device__ __forceinline__ Band xcd_band(int items)
     return Band{first, last - first, (int)(blockIdx.x >> 3), (int)(gridDim.x >> 3)};
 }
 
-// 32 output channels. KIND 0: the network input (4 channels, 8 bytes a
-// pixel), K packed as (kernel row, channel): chunk 0 holds rows 0..3, chunk 1
-// row 4 and zeros. KIND 1: 16 input channels. KIND 2: 32 input channels.
+// 32 output channels. KIND 1: 16 input channels. KIND 2: 32 input channels.
+// (The network input layer, 4 channels, is k_conv_in.)
 // EPI_HIDDEN: bias, GELU, optional pre-activation; EPI_DX: times gelu'(gpre).
 // Two wavefronts per SIMD (at most 256 registers): two workgroups share a CU.
 template <int KIND, int EPI>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_conv_sh(
     ConvArgs a)
 {
-    static_assert(EPI == EPI_HIDDEN || (EPI == EPI_DX && KIND != 0), "epilogue");
+    static_assert(KIND == 1 || KIND == 2, "16 or 32 input channels");
+    static_assert(EPI == EPI_HIDDEN || EPI == EPI_DX, "epilogue");
     constexpr int NC = KIND == 2 ? 2 : 1;
     constexpr int CIN = KIND == 2 ? 32 : 16;
     constexpr int ROW = CIN + 8;  // LDS row of one output channel, padded by 8
-    constexpr int NROWS = KIND == 0 ? 5 * 2 * 32 : TAPS * 32;
+    constexpr int NROWS = TAPS * 32;
     __shared__ __attribute__((aligned(16))) __bf16 wsh[NROWS * ROW];
     // Weight fill (256 threads): every global load is issued before the first
     // LDS store, so the workgroup waits for one memory latency, not one per
     // loop trip.
     const int tid = (int)threadIdx.x;
-    if constexpr (KIND == 0) {
-        // wsh[(dx * 2 + chunk) * 32 + co][4 kq .. 4 kq + 3] from the [TAPS][32][16]
-        // weights: chunk 0 holds kernel row kq, chunk 1 row 4 (kq = 0) and zeros
-        constexpr int NQ = 5 * 2 * 32 * 4 / 256;
-        bf16x4 v[NQ];
-#pragma unroll
-        for (int i = 0; i < NQ; i++) {
-            const int q = tid + 256 * i;
-            const int kq = q & 3, co = (q >> 2) & 31, ch = (q >> 7) & 1, dx = q >> 8;
-            const int dy = ch == 0 ? kq : 4;
-            v[i] = (ch == 0 || kq == 0)
-                       ? *reinterpret_cast<const bf16x4 *>(a.w + ((size_t)(dy * 5 + dx) * 32 + co) * 16)
-                       : bf16x4{};
-        }
-#pragma unroll
-        for (int i = 0; i < NQ; i++) {
-            const int q = tid + 256 * i;
-            const int kq = q & 3, co = (q >> 2) & 31, ch = (q >> 7) & 1, dx = q >> 8;
-            *reinterpret_cast<bf16x4 *>(&wsh[((dx * 2 + ch) * 32 + co) * ROW + kq * 4]) = v[i];
-        }
-    } else {
+    {
         constexpr int NTOT = TAPS * 32 * (CIN / 8), NQ = (NTOT + 255) / 256;
         bf16x8 v[NQ];
 #pragma unroll
@@ -164,7 +144,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 
     const int lane = (int)(threadIdx.x & 63), n = lane & 31, h = lane >> 5;
     const int nx = a.nx, Wp = nx + 4, T = (nx + 27) / 28;
-    constexpr int PXS = KIND == 0 ? IN0C : 16;  // elements per padded pixel of a plane
+    constexpr int PXS = 16;  // elements per padded pixel of a plane
     const size_t PL = plane_elems(nx, a.ny), RS = (size_t)Wp * PXS;  // row stride
     const Band bd = xcd_band(a.ny * T);
     const int wib = bd.local * 4 + (int)(threadIdx.x >> 6), nwb = bd.nlocal * 4;
@@ -223,7 +203,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     // current pair runs its MFMAs and epilogue.
     int k = 2 * wib;
     if (k >= bd.count) return;
-    const size_t coff = KIND == 0 ? 0 : (size_t)h * 8;
+    const size_t coff = (size_t)h * 8;
     // positions of the pair at k, advanced by 2 nwb items per pair (no
     // division in the loop)
     const int step = 2 * nwb, step_y = step / T, step_t = step - step_y * T;
@@ -246,110 +226,153 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         if (kk + 1 < bd.count) finish(acc[1], next_pos(p, 0, 1));  // wavefront-uniform
     };
 
-    if constexpr (KIND == 0) {
-        // rows 2h, 2h + 1 and 4 of each tile, channels 0..3
-        bf16x4 raw[2][3];
-        auto load = [&](const __bf16 *pa, const __bf16 *pb) __attribute__((always_inline)) {
-            raw[0][0] = *reinterpret_cast<const bf16x4 *>(pa + (2 * h) * RS);
-            raw[0][1] = *reinterpret_cast<const bf16x4 *>(pa + (2 * h + 1) * RS);
-            raw[0][2] = *reinterpret_cast<const bf16x4 *>(pa + 4 * RS);
-            raw[1][0] = *reinterpret_cast<const bf16x4 *>(pb + (2 * h) * RS);
-            raw[1][1] = *reinterpret_cast<const bf16x4 *>(pb + (2 * h + 1) * RS);
-            raw[1][2] = *reinterpret_cast<const bf16x4 *>(pb + 4 * RS);
-        };
-        load(sa, sb);
-        for (; k < bd.count; k += 2 * nwb) {
-            const Pos cur = pos;
-            const bf16x4 z = {};
-            bf16x8 B0[2], B1[2];
+    // Two register slots of one kernel row each: row dy of a pair sits in
+    // slot (dy + P) & 1, and row 0 of the next pair is loaded with row 4
+    // of this one into slot (P + 1) & 1, so consecutive pairs alternate P.
+    bf16x8 buf[2][NC][2];
+    auto load_row = [&](const __bf16 *pa, const __bf16 *pb, int dy, int slot)
+                        __attribute__((always_inline)) {
 #pragma unroll
-            for (int j = 0; j < 2; j++) {
-                B0[j] = __builtin_shufflevector(raw[j][0], raw[j][1], 0, 1, 2, 3, 4, 5, 6, 7);
-                B1[j] = __builtin_shufflevector(h ? z : raw[j][2], z, 0, 1, 2, 3, 4, 5, 6, 7);
-            }
-            if (k + 2 * nwb < bd.count) {
-                pos = next_pos(pos, step_y, step_t);
-                pair_src(pos, k + 2 * nwb, sa, sb);
-                load(sa, sb);
-            }
+        for (int c = 0; c < NC; c++) {
+            const size_t o = (size_t)c * PL + (size_t)dy * RS;
+            buf[slot][c][0] = *reinterpret_cast<const bf16x8 *>(pa + o);
+            buf[slot][c][1] = *reinterpret_cast<const bf16x8 *>(pb + o);
+        }
+    };
+    auto pair = [&](auto parity) __attribute__((always_inline)) {
+        constexpr int P = decltype(parity)::value;
+        const bool more = k + 2 * nwb < bd.count;  // wavefront-uniform
+        const __bf16 *na = sa, *nb = sb;
+        const Pos cur = pos;
+        if (more) {
+            pos = next_pos(pos, step_y, step_t);
+            pair_src(pos, k + 2 * nwb, na, nb);
+        }
 #pragma unroll
-            for (int j = 0; j < 2; j++)
+        for (int j = 0; j < 2; j++)
 #pragma unroll
-                for (int dx = 0; dx < 5; dx++) acc[j][dx] = f32x16{};
-            // chunk-outer order: an accumulator comes back every 10 MFMAs
+            for (int dx = 0; dx < 5; dx++) acc[j][dx] = f32x16{};
 #pragma unroll
-            for (int ch = 0; ch < 2; ch++)
+        for (int dy = 0; dy < 5; dy++) {
+            if (dy < 4)
+                load_row(sa, sb, dy + 1, (dy + 1 + P) & 1);
+            else if (more)
+                load_row(na, nb, 0, (P + 1) & 1);
+            __builtin_amdgcn_sched_barrier(0);
+            // channel-chunk-outer order: an accumulator comes back every
+            // 10 MFMAs, past the latency of the previous one
+#pragma unroll
+            for (int c = 0; c < NC; c++)
 #pragma unroll
                 for (int dx = 0; dx < 5; dx++) {
                     const bf16x8 w = *reinterpret_cast<const bf16x8 *>(
-                        wsh + ((dx * 2 + ch) * 32 + n) * ROW + h * 8);
-                    acc[0][dx] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, ch ? B1[0] : B0[0],
-                                                                         acc[0][dx], 0, 0, 0);
-                    acc[1][dx] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, ch ? B1[1] : B0[1],
-                                                                         acc[1][dx], 0, 0, 0);
+                        wsh + ((dy * 5 + dx) * 32 + n) * ROW + h * 8 + c * 16);
+                    acc[0][dx] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                        w, buf[(dy + P) & 1][c][0], acc[0][dx], 0, 0, 0);
+                    acc[1][dx] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                        w, buf[(dy + P) & 1][c][1], acc[1][dx], 0, 0, 0);
                 }
-            finish_pair(cur, k);
         }
-    } else {
-        // Two register slots of one kernel row each: row dy of a pair sits in
-        // slot (dy + P) & 1, and row 0 of the next pair is loaded with row 4
-        // of this one into slot (P + 1) & 1, so consecutive pairs alternate P.
-        bf16x8 buf[2][NC][2];
-        auto load_row = [&](const __bf16 *pa, const __bf16 *pb, int dy, int slot)
-                            __attribute__((always_inline)) {
+        finish_pair(cur, k);
+        k += 2 * nwb;
+        sa = na;
+        sb = nb;
+    };
+    load_row(sa, sb, 0, 0);
+    while (true) {
+        pair(std::integral_constant<int, 0>{});
+        if (k >= bd.count) break;
+        pair(std::integral_constant<int, 1>{});
+        if (k >= bd.count) break;
+    }
+}
+
+// Network input layer (4 -> 32 channels) as an implicit GEMM with the taps in
+// K: K index 4 tap + ci over the 25 taps and 4 channels (100, padded to 112 =
+// 7 chunks of 16), so one accumulator holds the whole 5x5 sum of a 32-pixel
+// tile and no horizontal shift-sum or tile overlap is needed. Lane (n, h)
+// of chunk c carries taps 4c + 2h and 4c + 2h + 1: as A, the weights of
+// output channel n at those taps (the 7 fragments stay in registers for the
+// whole launch); as B, the 4 input channels of pixel x0 + n at those taps
+// (two 8-byte loads). The layer's MFMA work is small and its epilogue (GELU
+// of 32 channels a pixel and 64 bytes of stores) large, so the kernel keeps
+// its registers under 128 for 4 wavefronts per SIMD instead of sharing one
+// loaded fragment among five accumulators. Tiles sit at x0 = min(32 t,
+// nx - 32); the last one overlaps its neighbour and stores only x >= 32 t.
+constexpr int IN_CHUNKS = 7;
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_conv_in(
+    ConvArgs a)
+{
+    const int lane = (int)(threadIdx.x & 63), n = lane & 31, h = lane >> 5;
+    const int nx = a.nx, Wp = nx + 4, T = (nx + 31) / 32;
+    const size_t PL = plane_elems(nx, a.ny);
+    // A fragments: W[tap][co = n][0..3] for taps 4c + 2h, 4c + 2h + 1
+    bf16x8 wf[IN_CHUNKS];
 #pragma unroll
-            for (int c = 0; c < NC; c++) {
-                const size_t o = (size_t)c * PL + (size_t)dy * RS;
-                buf[slot][c][0] = *reinterpret_cast<const bf16x8 *>(pa + o);
-                buf[slot][c][1] = *reinterpret_cast<const bf16x8 *>(pb + o);
+    for (int c = 0; c < IN_CHUNKS; c++) {
+        const int t0 = 4 * c + 2 * h, t1 = t0 + 1;
+        const bf16x4 z = {};
+        const bf16x4 w0 = t0 < TAPS ? *reinterpret_cast<const bf16x4 *>(
+                                          a.w + ((size_t)t0 * HID + n) * CIN0)
+                                    : z;
+        const bf16x4 w1 = t1 < TAPS ? *reinterpret_cast<const bf16x4 *>(
+                                          a.w + ((size_t)t1 * HID + n) * CIN0)
+                                    : z;
+        wf[c] = __builtin_shufflevector(w0, w1, 0, 1, 2, 3, 4, 5, 6, 7);
+    }
+    const Band bd = xcd_band(a.ny * T);
+    const int wib = bd.local * 4 + (int)(threadIdx.x >> 6), nwb = bd.nlocal * 4;
+    // the taps of chunk c for either half of the wavefront are compile-time
+    // constants; each load selects its half's (dy, dx) and skips taps >= 25
+    auto tap_load = [&](const __bf16 *src, int c, int q) __attribute__((always_inline)) {
+        const int t0 = 4 * c + q, t1 = 4 * c + 2 + q;  // h = 0, h = 1
+        const int tap = h ? t1 : t0;
+        const bf16x4 z = {};
+        if (tap >= TAPS) return z;
+        const int dy = h ? t1 / 5 : t0 / 5, dx = h ? t1 % 5 : t0 % 5;
+        return *reinterpret_cast<const bf16x4 *>(src + ((size_t)dy * Wp + dx) * IN0C);
+    };
+    auto load = [&](int item, bf16x8(&b)[IN_CHUNKS]) __attribute__((always_inline)) {
+        const int y = item / T, t = item - y * T, x0 = min(32 * t, nx - 32);
+        const __bf16 *src = a.in + ((size_t)y * Wp + x0 + n) * IN0C;
+#pragma unroll
+        for (int c = 0; c < IN_CHUNKS; c++)
+            b[c] = __builtin_shufflevector(tap_load(src, c, 0), tap_load(src, c, 1), 0, 1, 2,
+                                           3, 4, 5, 6, 7);
+    };
+    int k = wib;
+    if (k >= bd.count) return;
+    bf16x8 b[IN_CHUNKS], bn[IN_CHUNKS];
+    load(bd.first + k, b);
+    for (; k < bd.count; k += nwb) {
+        const int item = bd.first + k;
+        if (k + nwb < bd.count) load(item + nwb, bn);  // next item's fragments in flight
+        f32x16 acc = {};
+#pragma unroll
+        for (int c = 0; c < IN_CHUNKS; c++)
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wf[c], b[c], acc, 0, 0, 0);
+        const int y = item / T, t = item - y * T, x0 = min(32 * t, nx - 32);
+        const int x = x0 + n;
+        if (x >= 32 * t) {
+            const size_t px = pad_index(y, x, nx) * 16;
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                const int co = g * 8 + h * 4;
+                const size_t o = (size_t)(co >> 4) * PL + px + (co & 15);
+                bf16x4 p, r;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const float v = acc[4 * g + j] + a.bias[co + j];
+                    p[j] = (__bf16)v;
+                    r[j] = (__bf16)gelu(v);
+                }
+                if (a.pre) *reinterpret_cast<bf16x4 *>(a.pre + o) = p;
+                *reinterpret_cast<bf16x4 *>(a.out + o) = r;
             }
-        };
-        auto pair = [&](auto parity) __attribute__((always_inline)) {
-            constexpr int P = decltype(parity)::value;
-            const bool more = k + 2 * nwb < bd.count;  // wavefront-uniform
-            const __bf16 *na = sa, *nb = sb;
-            const Pos cur = pos;
-            if (more) {
-                pos = next_pos(pos, step_y, step_t);
-                pair_src(pos, k + 2 * nwb, na, nb);
-            }
-#pragma unroll
-            for (int j = 0; j < 2; j++)
-#pragma unroll
-                for (int dx = 0; dx < 5; dx++) acc[j][dx] = f32x16{};
-#pragma unroll
-            for (int dy = 0; dy < 5; dy++) {
-                if (dy < 4)
-                    load_row(sa, sb, dy + 1, (dy + 1 + P) & 1);
-                else if (more)
-                    load_row(na, nb, 0, (P + 1) & 1);
-                __builtin_amdgcn_sched_barrier(0);
-                // channel-chunk-outer order: an accumulator comes back every
-                // 10 MFMAs, past the latency of the previous one
-#pragma unroll
-                for (int c = 0; c < NC; c++)
-#pragma unroll
-                    for (int dx = 0; dx < 5; dx++) {
-                        const bf16x8 w = *reinterpret_cast<const bf16x8 *>(
-                            wsh + ((dy * 5 + dx) * 32 + n) * ROW + h * 8 + c * 16);
-                        acc[0][dx] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                            w, buf[(dy + P) & 1][c][0], acc[0][dx], 0, 0, 0);
-                        acc[1][dx] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                            w, buf[(dy + P) & 1][c][1], acc[1][dx], 0, 0, 0);
-                    }
-            }
-            finish_pair(cur, k);
-            k += 2 * nwb;
-            sa = na;
-            sb = nb;
-        };
-        load_row(sa, sb, 0, 0);
-        while (true) {
-            pair(std::integral_constant<int, 0>{});
-            if (k >= bd.count) break;
-            pair(std::integral_constant<int, 1>{});
-            if (k >= bd.count) break;
         }
+#pragma unroll
+        for (int c = 0; c < IN_CHUNKS; c++) b[c] = bn[c];
     }
 }
 
@@ -1016,10 +1039,11 @@ void launch_conv(hipStream_t s, int cin, int cout, Epilogue epi, const ConvArgs 
     if (a.nx < 32 || a.ny <= 0)
         NEMO_FATAL("closure conv: interior %d x %d (nx must be at least 32)", a.nx, a.ny);
     // k_conv_sh: persistent, 2 workgroups per CU (LDS weights and 256
+    // registers); k_conv_in: persistent, 4 workgroups per CU (128
     // registers); k_conv_out: one item per wavefront (out_grid)
     const dim3 grid = conv_grid(2), block(256);
     if (cin == 16 && cout == 32 && epi == EPI_HIDDEN)  // the network input: 4 used channels
-        k_conv_sh<0, EPI_HIDDEN><<<grid, block, 0, s>>>(a);
+        k_conv_in<<<conv_grid(4), block, 0, s>>>(a);
     else if (cin == 32 && cout == 32 && epi == EPI_HIDDEN)
         k_conv_sh<2, EPI_HIDDEN><<<grid, block, 0, s>>>(a);
     else if (cin == 32 && cout == 16 && epi == EPI_OUTPUT)
