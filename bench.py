@@ -38,6 +38,20 @@ METRIC = ("grid-pt updates/s + % HBM peak at 4096²; "
 HBM_PEAK_GBS = 8000.0
 B_ALG_F64 = 212          # bytes per interior grid-point update, FP64
 B_ALG_MIXED = 113        # FP32 fields, the byte mask and the ssh compensation word
+# nemo_hip_fused_variant_shape's march kind -> description
+_ALIGNED = "overlapped-wavefront march, 56 columns, 64-byte-aligned stores"
+MARCH_KINDS = {
+    0: "row march", 1: "row march, loads one row ahead",
+    2: "overlapped-wavefront march", 3: "overlapped-wavefront march, >= 3 waves per SIMD",
+    4: "overlapped-wavefront march, >= 4 waves per SIMD",
+    5: _ALIGNED,
+    6: "overlapped-wavefront march, 48 columns, 64-byte-aligned stores",
+    7: _ALIGNED + ", loads two rows ahead",
+    8: _ALIGNED + ", loads two rows ahead, >= 3 waves per SIMD",
+    9: "persistent chunked march, 56 columns, 64-byte-aligned stores, loads two rows ahead",
+    10: "persistent chunked march, 56 columns, 64-byte-aligned stores",
+    11: _ALIGNED + ", loads three rows ahead",
+}
 # rocprofv3 summaries holding "[label] traffic_bytes_per_launch:" lines
 PROFILE_TRAFFIC = (ROOT / "profiles" / "fused_step_4096_counters.md",
                    ROOT / "profiles" / "ensemble_step_256_domains_counters.md",
@@ -412,15 +426,7 @@ def main():
             "hbm_frac_whole_step": B_ALG_F64 * updates_per_s / (HBM_PEAK_GBS * 1e9),
             "fused_variant": best,
             "fused_shape": {"block_columns": bw.value, "rows": rows.value,
-                            "kind": ("persistent chunked march, 56 columns, 64-byte-aligned stores"
-                                     + (", loads two rows ahead" if pf.value == 9 else "")
-                                     if pf.value >= 9 else
-                                     "overlapped-wavefront march, 56 columns, 64-byte-aligned "
-                                     "stores, loads two rows ahead" if pf.value >= 7 else
-                                     f"overlapped-wavefront march, {48 if pf.value == 6 else 56} "
-                                     "columns, 64-byte-aligned stores" if pf.value >= 5 else
-                                     ["row march", "row march, loads one row ahead",
-                                      "overlapped-wavefront march"][min(pf.value, 2)])},
+                            "kind": MARCH_KINDS[pf.value]},
             "variant_kernel_ms": {str(k): round(v, 4) for k, v in tune.items()},
         },
         "roofline": {

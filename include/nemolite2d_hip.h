@@ -178,7 +178,8 @@ NEMO_API int nemo_hip_fused_variant(void);
  * stores), 7 / 8 = the 56-column march with its loads issued two rows
  * ahead, compiled for at least 2 / 3 waves per SIMD, 9 / 10 = the persistent
  * chunked 56-column march with / without the loads two rows ahead (rows = 0:
- * one chunk of rows per wavefront the GPU holds at once, set at launch). */
+ * one chunk of rows per wavefront the GPU holds at once, set at launch),
+ * 11 = the 56-column march with its loads issued three rows ahead. */
 NEMO_API void nemo_hip_fused_variant_shape(int variant, int *block_width,
                                            int *rows, int *prefetch);
 /* Name of the kernel a variant launches, as profilers report it. */
