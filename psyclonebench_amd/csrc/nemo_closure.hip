@@ -288,18 +288,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 }
 
 // Network input layer (4 -> 32 channels) as an implicit GEMM with the taps in
-// K: K index 4 tap + ci over the 25 taps and 4 channels (100, padded to 112 =
-// 7 chunks of 16), so one accumulator holds the whole 5x5 sum of a 32-pixel
-// tile and no horizontal shift-sum or tile overlap is needed. Lane (n, h)
-// of chunk c carries taps 4c + 2h and 4c + 2h + 1: as A, the weights of
-// output channel n at those taps (the 7 fragments stay in registers for the
-// whole launch); as B, the 4 input channels of pixel x0 + n at those taps
-// (two 8-byte loads). The layer's MFMA work is small and its epilogue (GELU
-// of 32 channels a pixel and 64 bytes of stores) large, so the kernel keeps
-// its registers under 128 for 4 wavefronts per SIMD instead of sharing one
-// loaded fragment among five accumulators. Tiles sit at x0 = min(32 t,
-// nx - 32); the last one overlaps its neighbour and stores only x >= 32 t.
+// K: 25 taps x 4 channels (100, padded to 112 = 7 chunks of 16), so one
+// accumulator holds the whole 5x5 sum of a 32-pixel tile and no horizontal
+// shift-sum or tile overlap is needed. Each half h of the wavefront carries
+// two taps of a chunk (8 K values): chunk c < 5 holds kernel row c, columns
+// 2h and 2h + 1; chunk 5 column 4 of rows 2h and 2h + 1; chunk 6 tap (4, 4)
+// in half 0 and zeros. As A, lane (n, h) holds the weights of output
+// channel n at its two taps (the 7 fragments stay in registers for the
+// whole launch); as B, the 4 input channels of pixel x0 + n at those taps:
+// two 8-byte loads whose addresses differ between chunks by a
+// wavefront-uniform row offset. The layer's MFMA work is small and its
+// epilogue (GELU of 32 channels a pixel and 64 bytes of stores) large, so
+// the kernel keeps its registers under 128 for 4 wavefronts per SIMD
+// instead of sharing one loaded fragment among five accumulators. Tiles sit
+// at x0 = min(32 t, nx - 32); the last one overlaps its neighbour and stores
+// only x >= 32 t.
 constexpr int IN_CHUNKS = 7;
+
+// Kernel tap (dy * 5 + dx) of chunk c, half h, slot q; -1: a zero K slot.
+__device__ __forceinline__ int in_tap(int c, int h, int q)
+{
+    if (c < 5) return c * 5 + 2 * h + q;
+    if (c == 5) return (2 * h + q) * 5 + 4;
+    return h == 0 && q == 0 ? 24 : -1;
+}
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_conv_in(
     ConvArgs a)
@@ -307,39 +319,43 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     const int lane = (int)(threadIdx.x & 63), n = lane & 31, h = lane >> 5;
     const int nx = a.nx, Wp = nx + 4, T = (nx + 31) / 32;
     const size_t PL = plane_elems(nx, a.ny);
-    // A fragments: W[tap][co = n][0..3] for taps 4c + 2h, 4c + 2h + 1
+    // A fragments: W[tap][co = n][0..3] at this half's two taps of each chunk
     bf16x8 wf[IN_CHUNKS];
 #pragma unroll
     for (int c = 0; c < IN_CHUNKS; c++) {
-        const int t0 = 4 * c + 2 * h, t1 = t0 + 1;
         const bf16x4 z = {};
-        const bf16x4 w0 = t0 < TAPS ? *reinterpret_cast<const bf16x4 *>(
-                                          a.w + ((size_t)t0 * HID + n) * CIN0)
-                                    : z;
-        const bf16x4 w1 = t1 < TAPS ? *reinterpret_cast<const bf16x4 *>(
-                                          a.w + ((size_t)t1 * HID + n) * CIN0)
-                                    : z;
-        wf[c] = __builtin_shufflevector(w0, w1, 0, 1, 2, 3, 4, 5, 6, 7);
+        bf16x4 w[2];
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const int tap = in_tap(c, h, q);
+            w[q] = tap >= 0 ? *reinterpret_cast<const bf16x4 *>(
+                                  a.w + ((size_t)tap * HID + n) * CIN0)
+                            : z;
+        }
+        wf[c] = __builtin_shufflevector(w[0], w[1], 0, 1, 2, 3, 4, 5, 6, 7);
     }
     const Band bd = xcd_band(a.ny * T);
     const int wib = bd.local * 4 + (int)(threadIdx.x >> 6), nwb = bd.nlocal * 4;
-    // the taps of chunk c for either half of the wavefront are compile-time
-    // constants; each load selects its half's (dy, dx) and skips taps >= 25
-    auto tap_load = [&](const __bf16 *src, int c, int q) __attribute__((always_inline)) {
-        const int t0 = 4 * c + q, t1 = 4 * c + 2 + q;  // h = 0, h = 1
-        const int tap = h ? t1 : t0;
-        const bf16x4 z = {};
-        if (tap >= TAPS) return z;
-        const int dy = h ? t1 / 5 : t0 / 5, dx = h ? t1 % 5 : t0 % 5;
-        return *reinterpret_cast<const bf16x4 *>(src + ((size_t)dy * Wp + dx) * IN0C);
-    };
+    const int RS = Wp * IN0C;  // elements per padded input row
     auto load = [&](int item, bf16x8(&b)[IN_CHUNKS]) __attribute__((always_inline)) {
         const int y = item / T, t = item - y * T, x0 = min(32 * t, nx - 32);
-        const __bf16 *src = a.in + ((size_t)y * Wp + x0 + n) * IN0C;
+        // padded pixel (y, x0 + n): the tile's tap (0, 0) for this lane
+        const __bf16 *px = a.in + ((size_t)y * Wp + x0 + n) * IN0C;
+        const __bf16 *rows = px + 2 * h * IN0C;          // chunks 0..4: columns 2h, 2h + 1
+        const __bf16 *col4 = px + 4 * IN0C + 2 * h * RS;  // chunk 5: column 4, rows 2h, 2h + 1
+        // two 8-byte loads a chunk: a pixel's 4 channels are 8-byte aligned only
 #pragma unroll
-        for (int c = 0; c < IN_CHUNKS; c++)
-            b[c] = __builtin_shufflevector(tap_load(src, c, 0), tap_load(src, c, 1), 0, 1, 2,
-                                           3, 4, 5, 6, 7);
+        for (int c = 0; c < 5; c++) {
+            const bf16x4 u0 = *reinterpret_cast<const bf16x4 *>(rows + c * RS);
+            const bf16x4 u1 = *reinterpret_cast<const bf16x4 *>(rows + c * RS + IN0C);
+            b[c] = __builtin_shufflevector(u0, u1, 0, 1, 2, 3, 4, 5, 6, 7);
+        }
+        const bf16x4 r0 = *reinterpret_cast<const bf16x4 *>(col4);
+        const bf16x4 r1 = *reinterpret_cast<const bf16x4 *>(col4 + RS);
+        b[5] = __builtin_shufflevector(r0, r1, 0, 1, 2, 3, 4, 5, 6, 7);
+        const bf16x4 z = {};
+        const bf16x4 t44 = *reinterpret_cast<const bf16x4 *>(px + 4 * RS + 4 * IN0C);
+        b[6] = __builtin_shufflevector(h ? z : t44, z, 0, 1, 2, 3, 4, 5, 6, 7);
     };
     int k = wib;
     if (k >= bd.count) return;
