@@ -55,6 +55,8 @@ MARCH_KINDS = {
 # rocprofv3 summaries holding "[label] traffic_bytes_per_launch:" lines
 PROFILE_TRAFFIC = (ROOT / "profiles" / "fused_step_4096_counters.md",
                    ROOT / "profiles" / "ensemble_step_256_domains_counters.md",
+                   ROOT / "profiles" / "mixed_step_4096_prefetch2_counters.md",
+                   ROOT / "profiles" / "ensemble_step_256_domains_prefetch2_counters.md",
                    ROOT / "profiles" / "step_kernels_4096_traffic.md")
 
 
@@ -160,7 +162,7 @@ def mixed_leg(pristine, a, pts, f64_ms_per_step, f64_kernel_ms, f64_variant) -> 
     tune = {}
     for cols in (61, 56, 48):
         lib.nemo_mp_set_columns(st.handle, cols)
-        for pf in (0, 1):
+        for pf in (0, 1, 2):
             for rows in ((4, 8) if pf else (4, 8, 16, 32, 48, 64, 80, 96, 128)):
                 lib.nemo_mp_set_prefetch(st.handle, 0)
                 lib.nemo_mp_set_rows(st.handle, rows)
@@ -207,15 +209,15 @@ def mixed_leg(pristine, a, pts, f64_ms_per_step, f64_kernel_ms, f64_variant) -> 
         "kernel_speedup_vs_f64": f64_kernel_ms / kernel_ms,
         "rows_per_band": rows,
         "columns_per_wavefront": cols,
-        "loads_two_rows_ahead": bool(pf),
-        "rows_kernel_ms": {f"{r}x{c}" + ("p" if q else ""): round(v, 4)
+        "loads_issued_rows_ahead": pf + 1,
+        "rows_kernel_ms": {f"{r}x{c}" + (f"p{q}" if q else ""): round(v, 4)
                            for (r, c, q), v in tune.items()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic_from_profile(
                          f"mixed rows {rows}" + ("" if cols == 61 else f" cols {cols}")
-                         + (" prefetch" if pf else "")),
-                     "kernel": (f"k_mixed_wave_p<{rows}, {cols}>" if pf else
+                         + ("" if not pf else " prefetch" if pf == 1 else f" prefetch {pf}")),
+                     "kernel": (f"k_mixed_wave_p<{rows}, {cols}, {pf}>" if pf else
                                 f"k_mixed_wave<{rows}>" if cols == 61
                                 else f"k_mixed_wave_a<{rows}, {cols}>"),
                      "kernel_ms": kernel_ms,

@@ -211,9 +211,9 @@ NEMO_API void nemo_mp_set_rows(void *handle, int rows);
  * 56 (every strip's stores start on a 32-byte boundary) or 48 (on a 64-byte
  * boundary). */
 NEMO_API void nemo_mp_set_columns(void *handle, int cols);
-/* 1: the kernel that issues its loads two rows ahead (rows per band 4 or 8
- * only); 0 (default): the loads of a row issued where the row starts. Both
- * compute the same bits. */
+/* 1 / 2: the kernel that issues its loads two / three rows ahead (rows per
+ * band 4 or 8 only); 0 (default): the loads of a row issued where the row
+ * starts. All compute the same bits. */
 NEMO_API void nemo_mp_set_prefetch(void *handle, int prefetch);
 /* Blocking read-back as FP64 (total_size values): field 0 sshn_t (hi+lo),
  * 1 un, 2 vn, 3 sshn_u, 4 sshn_v. */
@@ -246,7 +246,8 @@ NEMO_API void *nemo_ens_create(int ndom, void *const *arrays, const int *geom,
  * width (owned columns per wavefront: 61, or 56 for 64-byte-aligned stores,
  * or 0: per domain, whichever of the two its width runs faster with) and
  * nemo_ens_shape_prefetch the rows of loads the tile issues ahead of the row
- * that needs them, beyond the next one (0 or 1). */
+ * that needs them, beyond the next one (0, 1, or 2 with one workgroup per
+ * tile). */
 NEMO_API int nemo_ens_shape_count(void *handle);
 NEMO_API void nemo_ens_shape_info(void *handle, int shape, int *rows,
                                   int *waves_per_block, int *blocks);

@@ -42,7 +42,8 @@ struct EnsShape {
                            // width runs faster with (one wavefront per tile
                            // only)
     int prefetch;          // rows of loads issued ahead of the row that needs
-                           // them, beyond the next: 0 or 1
+                           // them, beyond the next: 0, 1 or 2 (2: one
+                           // wavefront per tile only)
 };
 
 // True when a kernel instantiation exists for the shape's launch form, rows,

@@ -144,7 +144,7 @@ void check_shape(const EnsShape &sh)
                    sh.waves_per_block);
     if (sh.cols != 61 && sh.cols != 56 && sh.cols != 0)
         NEMO_FATAL("ensemble launch: %d columns per wavefront", sh.cols);
-    if (sh.prefetch != 0 && sh.prefetch != 1)
+    if (sh.prefetch < 0 || sh.prefetch > 2)
         NEMO_FATAL("ensemble launch: prefetch depth %d", sh.prefetch);
 }
 
@@ -157,7 +157,8 @@ void check_shape(const EnsShape &sh)
 #define NEMO_ENS_GRID(X)                                                               \
     X(4, 61, 0) X(8, 61, 0) X(16, 61, 0) X(4, 56, 0) X(8, 56, 0) X(16, 56, 0)          \
     X(4, 61, 1) X(8, 61, 1) X(16, 61, 1) X(4, 56, 1) X(8, 56, 1) X(16, 56, 1)          \
-    X(4, 0, 0) X(8, 0, 0) X(4, 0, 1) X(8, 0, 1)
+    X(4, 0, 0) X(8, 0, 0) X(4, 0, 1) X(8, 0, 1) X(4, 61, 2) X(8, 61, 2) X(4, 56, 2)     \
+    X(8, 56, 2) X(4, 0, 2) X(8, 0, 2)
 
 void unsupported(const EnsShape &sh)
 {

@@ -229,7 +229,7 @@ void build_shapes(EnsContext &C)
     const int cus = prop.multiProcessorCount;
     for (int cols : kEnsCols)
         for (int rows : kEnsRows)
-            for (int pf : {0, 1}) {
+            for (int pf : {0, 1, 2}) {
                 // one wavefront per tile
                 const EnsShape grid{rows, 1, 0, cols, pf};
                 if (ens_shape_supported(grid)) C.shapes.push_back(grid);
@@ -534,13 +534,12 @@ void *nemo_ens_create(int ndom, void *const *arrays, const int *geom,
     build_flather_lists(*C, arrays, s);
     build_shapes(*C);
     build_lists(*C, s);
-    // default: one wavefront per tile, 4 rows x 56 columns, loads two rows
-    // ahead in FP64 and one in FP32 (the fastest shapes on a 256-domain
-    // ensemble of sides 64..1024 on MI355X)
-    const int pf = precision == 64 ? 1 : 0;
+    // default: one wavefront per tile, 4 rows, 61 or 56 columns per domain,
+    // loads three rows ahead, in both precisions (the fastest shape on a
+    // 256-domain ensemble of sides 64..1024 on MI355X)
     for (size_t k = 0; k < C->shapes.size(); k++) {
         const EnsShape &sh = C->shapes[k];
-        if (sh.blocks == 0 && sh.rows == 4 && sh.cols == 56 && sh.prefetch == pf)
+        if (sh.blocks == 0 && sh.rows == 4 && sh.cols == 0 && sh.prefetch == 2)
             C->shape = (int)k;
     }
     NEMO_HIP_CHECK(hipStreamSynchronize(s));

@@ -47,9 +47,9 @@ __global__ __launch_bounds__(64) void k_mixed_wave_a(MixedIn f, StepScalars<floa
                             wave::split(tide_tab ? tide_tab[tide_k] : tide_val));
 }
 
-// The OW-column march with the loads issued two rows ahead (tile_mixed's
-// PF = 1; OW = 61, 56 or 48).
-template <int R, int OW>
+// The OW-column march with the loads issued PF + 1 rows ahead (tile_mixed's
+// PF = 1 or 2; OW = 61, 56 or 48).
+template <int R, int OW, int PF = 1>
 __global__ __launch_bounds__(64) void k_mixed_wave_p(MixedIn f, StepScalars<float> p,
                                                      MixedOut o, int W, Bounds b,
                                                      int nstrip,
@@ -58,7 +58,7 @@ __global__ __launch_bounds__(64) void k_mixed_wave_p(MixedIn f, StepScalars<floa
 {
     const int logical = wave::xcd_tile((int)blockIdx.x, (int)gridDim.x);
     const int band = logical / nstrip, strip = logical - band * nstrip;
-    wave::tile_mixed<R, OW, 1>(f, p, o, W, 0u, b, strip, band,
+    wave::tile_mixed<R, OW, PF>(f, p, o, W, 0u, b, strip, band,
                                wave::split(tide_tab ? tide_tab[tide_k] : tide_val));
 }
 
@@ -131,25 +131,29 @@ void launch_mixed_step(hipStream_t s, int rows, int cols, int prefetch, const Mi
     const int ncol = b.xe - b.xs + 3, nrow = b.ye - b.ys + 3;
     const int nstrip = (ncol + cols - 1) / cols;
     if (prefetch) {
-        switch (rows) {
-#define NEMO_MIXED_PF_CASE(RR)                                                       \
-    case RR: {                                                                       \
+        switch (rows * 4 + prefetch) {
+#define NEMO_MIXED_PF_CASE(RR, PF)                                                   \
+    case RR * 4 + PF: {                                                              \
         const dim3 grid((unsigned)(nstrip * ((nrow + RR - 1) / RR)));                \
         if (cols == 48)                                                              \
-            k_mixed_wave_p<RR, 48><<<grid, dim3(64), 0, s>>>(                        \
+            k_mixed_wave_p<RR, 48, PF><<<grid, dim3(64), 0, s>>>(                    \
                 f, p, o, W, b, nstrip, tide_tab, tide_k, tide_val);                  \
         else if (cols == 56)                                                         \
-            k_mixed_wave_p<RR, 56><<<grid, dim3(64), 0, s>>>(                        \
+            k_mixed_wave_p<RR, 56, PF><<<grid, dim3(64), 0, s>>>(                    \
                 f, p, o, W, b, nstrip, tide_tab, tide_k, tide_val);                  \
         else                                                                         \
-            k_mixed_wave_p<RR, wave::OUT><<<grid, dim3(64), 0, s>>>(                 \
+            k_mixed_wave_p<RR, wave::OUT, PF><<<grid, dim3(64), 0, s>>>(             \
                 f, p, o, W, b, nstrip, tide_tab, tide_k, tide_val);                  \
         break;                                                                       \
     }
-            NEMO_MIXED_PF_CASE(4)
-            NEMO_MIXED_PF_CASE(8)
+            NEMO_MIXED_PF_CASE(4, 1)
+            NEMO_MIXED_PF_CASE(8, 1)
+            NEMO_MIXED_PF_CASE(4, 2)
+            NEMO_MIXED_PF_CASE(8, 2)
 #undef NEMO_MIXED_PF_CASE
-        default: NEMO_FATAL("mixed step: the prefetching kernel takes 4 or 8 rows, not %d", rows);
+        default:
+            NEMO_FATAL("mixed step: the prefetching kernel takes 4 or 8 rows and depth 1 or 2, "
+                       "not %d rows, depth %d", rows, prefetch);
         }
         NEMO_LAUNCH_CHECK();
         return;

@@ -22,7 +22,7 @@ struct MixedContext {
     double amp = 0.0, omega_tide = 0.0;
     int rows = 32;
     int cols = 61;                   // output columns per wavefront
-    int prefetch = 0;                // 1: loads issued two rows ahead (rows 4 or 8)
+    int prefetch = 0;                // 1 / 2: loads issued two / three rows ahead (rows 4 or 8)
     float *cst[A_COUNT] = {};        // constant arrays (FP32); tmask in tm
     signed char *tm = nullptr;       // T mask, one byte a point
     float *fcor_u = nullptr, *fcor_v = nullptr;
@@ -239,8 +239,8 @@ void nemo_mp_set_columns(void *h, int cols)
 void nemo_mp_set_prefetch(void *h, int prefetch)
 {
     MixedContext &C = *static_cast<MixedContext *>(h);
-    if (prefetch != 0 && prefetch != 1)
-        NEMO_FATAL("mixed form: prefetch must be 0 or 1 (got %d)", prefetch);
+    if (prefetch < 0 || prefetch > 2)
+        NEMO_FATAL("mixed form: prefetch must be 0, 1 or 2 (got %d)", prefetch);
     if (prefetch && C.rows != 4 && C.rows != 8)
         NEMO_FATAL("mixed form: the prefetching kernel takes 4 or 8 rows per band "
                    "(rows are %d)", C.rows);

@@ -41,7 +41,7 @@ def main() -> None:
                    help="output columns per wavefront of the mixed kernel (61, 48), "
                         "comma separated")
     p.add_argument("--mixed-prefetch", type=int, default=0,
-                   help="rows of loads the mixed kernel issues ahead (0 or 1)")
+                   help="extra rows of loads the mixed kernel keeps in flight (0, 1 or 2)")
     p.add_argument("--ensemble", type=int, default=0,
                    help="ensemble steps (plain launches) per precision and shape")
     p.add_argument("--ens-domains", type=int, default=256)

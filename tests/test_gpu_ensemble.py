@@ -71,7 +71,7 @@ def test_ensemble_every_shape_bitwise_equal(precision):
     nshape = len(probe.shapes())
     shapes = probe.shapes()
     assert {s[3] for s in shapes} == {61, 56, 0}
-    assert {s[4] for s in shapes} == {0, 1}
+    assert {s[4] for s in shapes} == {0, 1, 2}
     assert any(s[2] == 0 for s in shapes) and any(s[2] > 0 for s in shapes)
     probe.release()
     ref = None

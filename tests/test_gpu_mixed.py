@@ -80,7 +80,7 @@ def test_mixed_band_shapes_bitwise_equal():
     m = live_state_model(509, seed=54, random_mask=True)
     shapes = [(rows, cols, 0) for cols in (61, 56, 48)
               for rows in (4, 8, 16, 32, 48, 64, 80, 96, 128)]
-    shapes += [(rows, cols, 1) for cols in (61, 56, 48) for rows in (4, 8)]
+    shapes += [(rows, cols, pf) for pf in (1, 2) for cols in (61, 56, 48) for rows in (4, 8)]
     outs = []
     for rows, cols, pf in shapes:
         st = MixedStepper(m.copy(), rows=rows, cols=cols, prefetch=pf)
