@@ -360,6 +360,13 @@ NEMO_API int nemo_cl_guard_trips(void *handle);
 /* HIP events around the forward pass of later calls; ms of the last one */
 NEMO_API void nemo_cl_set_timing(void *handle, int on);
 NEMO_API double nemo_cl_forward_ms(void *handle);
+/* Launch form of the 32 -> 32 convolutions and their input gradients, for
+ * every closure in the process: 0 two 32-pixel tiles per wavefront pass,
+ * 4 wavefronts per workgroup; 1 (default) one tile, 8 wavefronts per
+ * workgroup. Both compute the same bits. Graphs nemo_cl_steps captured
+ * under the other form are not replayed. No reference counterpart. */
+NEMO_API void nemo_cl_set_conv_form(int form);
+NEMO_API int nemo_cl_conv_form(void);
 /* test hooks: forward / loss gradient of normalised host inputs
  * [ny*nx][4] and targets [ny*nx][3]; outputs [ny*nx][3], gradients in the
  * parameter layout; returns the loss */

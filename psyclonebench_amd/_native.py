@@ -159,6 +159,8 @@ def hip_lib() -> C.CDLL:
         lib.nemo_cl_set_timing.argtypes = [vp, C.c_int]
         lib.nemo_cl_forward_ms.argtypes = [vp]
         lib.nemo_cl_forward_ms.restype = C.c_double
+        lib.nemo_cl_set_conv_form.argtypes = [C.c_int]
+        lib.nemo_cl_conv_form.restype = C.c_int
         lib.nemo_cl_forward_host.argtypes = [vp, fp, fp]
         lib.nemo_cl_grad_host.argtypes = [vp, fp, fp, fp]
         lib.nemo_cl_grad_host.restype = C.c_double

@@ -105,6 +105,17 @@ def _state_ptrs(st: TimeStepper) -> tuple:
     return tuple(v.value for v in vals)
 
 
+def set_conv_form(form: int) -> None:
+    """Launch form of the 32 -> 32 convolutions for every network in the
+    process (nemo_cl_set_conv_form): 0 two 32-pixel tiles per wavefront
+    pass, 1 one tile with twice the wavefronts per workgroup. Same bits."""
+    hip_lib().nemo_cl_set_conv_form(int(form))
+
+
+def conv_form() -> int:
+    return hip_lib().nemo_cl_conv_form()
+
+
 class ClosureNet:
     """The network and its buffers for one coarse grid."""
 

@@ -531,6 +531,9 @@ def main(argv=None) -> None:
     s.add_argument("--weights", default=str(DEFAULT_WEIGHTS))
     s.add_argument("--steps", type=int, default=200,
                    help="coarse steps timed (the fine run covers the same model time)")
+    s.add_argument("--conv-form", type=int, default=None,
+                   help="launch form of the 32 -> 32 convolutions (0 or 1; default: the "
+                        "library's)")
     a = p.parse_args(argv)
     if a.cmd == "train":
         train(a)
@@ -542,6 +545,9 @@ def main(argv=None) -> None:
         if a.json:
             Path(a.json).write_text(json.dumps(res, indent=1))
     else:
+        if a.conv_form is not None:
+            from .closure import set_conv_form
+            set_conv_form(a.conv_form)
         log(json.dumps(throughput(a.n, a.factor, a.weights, steps=a.steps)))
 
 

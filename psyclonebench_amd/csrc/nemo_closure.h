@@ -74,6 +74,11 @@ struct ConvArgs {
 };
 
 void launch_conv(hipStream_t s, int cin, int cout, Epilogue epi, const ConvArgs &a);
+// Launch form of the 16/32 -> 32 convolutions (process-wide): 0, two tiles
+// per wavefront pass (4 wavefronts a workgroup); 1 (default), one tile (8
+// wavefronts). Both give the same bits.
+void set_conv_form(int form);
+int conv_form();
 // dW partial sums: partial[block][TAPS][co][ci] over the pixels of each of
 // the dw_blocks(nx, ny) workgroups
 int dw_blocks(int nx, int ny);

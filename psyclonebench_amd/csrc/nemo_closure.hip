@@ -109,33 +109,49 @@ __device__ __forceinline__ Band xcd_band(int items)
 // 32 output channels. KIND 1: 16 input channels. KIND 2: 32 input channels.
 // (The network input layer, 4 channels, is k_conv_in.)
 // EPI_HIDDEN: bias, GELU, optional pre-activation; EPI_DX: times gelu'(gpre).
-// Two wavefronts per SIMD (at most 256 registers): two workgroups share a CU.
-template <int KIND, int EPI>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_conv_sh(
-    ConvArgs a)
+// NT tiles per wavefront pass, WPB wavefronts per workgroup, two workgroups
+// per CU (the LDS weights):
+// - NT = 2, WPB = 4: two wavefronts per SIMD (at most 256 registers), each
+//   weight fragment read from LDS once for two tiles;
+// - NT = 1, WPB = 8: four wavefronts per SIMD (at most 128 registers), one
+//   LDS weight read per MFMA, twice the wavefronts to hide the activation
+//   loads and overlap one wavefront's epilogue with another's MFMAs.
+// Every tile sees the same MFMA sequence in both forms: the same bits.
+template <int KIND, int EPI, int NT, int WPB>
+__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(4 / NT, 4 / NT)))
+void k_conv_sh(ConvArgs a)
 {
     static_assert(KIND == 1 || KIND == 2, "16 or 32 input channels");
     static_assert(EPI == EPI_HIDDEN || EPI == EPI_DX, "epilogue");
+    static_assert((NT == 2 && WPB == 4) || (NT == 1 && WPB == 8), "launch form");
     constexpr int NC = KIND == 2 ? 2 : 1;
     constexpr int CIN = KIND == 2 ? 32 : 16;
     constexpr int ROW = CIN + 8;  // LDS row of one output channel, padded by 8
     constexpr int NROWS = TAPS * 32;
     __shared__ __attribute__((aligned(16))) __bf16 wsh[NROWS * ROW];
-    // Weight fill (256 threads): every global load is issued before the first
-    // LDS store, so the workgroup waits for one memory latency, not one per
-    // loop trip.
+    // the bias sits in LDS too: an LDS read in the epilogue waits on the LDS
+    // counter only, where a global load would wait for every store and
+    // every prefetched fragment before it
+    __shared__ __attribute__((aligned(16))) float bsh[32];
+    // Weight fill (every thread of the workgroup): every global load is
+    // issued before the first LDS store, so the workgroup waits for one
+    // memory latency, not one per loop trip.
     const int tid = (int)threadIdx.x;
+    if constexpr (EPI == EPI_HIDDEN) {
+        if (tid < 32) bsh[tid] = a.bias[tid];
+    }
     {
-        constexpr int NTOT = TAPS * 32 * (CIN / 8), NQ = (NTOT + 255) / 256;
+        constexpr int NTHR = 64 * WPB;
+        constexpr int NTOT = TAPS * 32 * (CIN / 8), NQ = (NTOT + NTHR - 1) / NTHR;
         bf16x8 v[NQ];
 #pragma unroll
         for (int i = 0; i < NQ; i++) {
-            const int q = tid + 256 * i;
+            const int q = tid + NTHR * i;
             if (q < NTOT) v[i] = *reinterpret_cast<const bf16x8 *>(a.w + (size_t)q * 8);
         }
 #pragma unroll
         for (int i = 0; i < NQ; i++) {
-            const int q = tid + 256 * i;
+            const int q = tid + NTHR * i;
             const int r = q / (CIN / 8), c8 = q - r * (CIN / 8);
             if (q < NTOT) *reinterpret_cast<bf16x8 *>(&wsh[r * ROW + c8 * 8]) = v[i];
         }
@@ -147,7 +163,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     constexpr int PXS = 16;  // elements per padded pixel of a plane
     const size_t PL = plane_elems(nx, a.ny), RS = (size_t)Wp * PXS;  // row stride
     const Band bd = xcd_band(a.ny * T);
-    const int wib = bd.local * 4 + (int)(threadIdx.x >> 6), nwb = bd.nlocal * 4;
+    const int wib = bd.local * WPB + (int)(threadIdx.x >> 6), nwb = bd.nlocal * WPB;
 
     // (the lambdas are forced inline: an out-of-line call would move the
     // accumulators to scratch memory)
@@ -180,10 +196,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             const size_t o = (size_t)(co >> 4) * PL + px + (co & 15);
             bf16x4 r;
             if constexpr (EPI == EPI_HIDDEN) {
+                const f32x4 bias = *reinterpret_cast<const f32x4 *>(&bsh[co]);
                 bf16x4 p;
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
-                    const float v = s[4 * g + k] + a.bias[co + k];
+                    const float v = s[4 * g + k] + bias[k];
                     p[k] = (__bf16)v;
                     r[k] = (__bf16)gelu(v);
                 }
@@ -198,87 +215,99 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         }
     };
 
-    // A wavefront takes the tile pairs at band offsets k = 2 wib, 2 (wib + nwb),
-    // ...; the activation fragments of the next pair are in flight while the
-    // current pair runs its MFMAs and epilogue.
-    int k = 2 * wib;
+    // A wavefront takes the groups of NT consecutive tiles at band offsets
+    // k = NT wib, NT (wib + nwb), ...; the activation fragments of the next
+    // group are in flight while the current group runs its MFMAs and
+    // epilogue.
+    int k = NT * wib;
     if (k >= bd.count) return;
     const size_t coff = (size_t)h * 8;
-    // positions of the pair at k, advanced by 2 nwb items per pair (no
+    // positions of the group at k, advanced by NT nwb items per group (no
     // division in the loop)
-    const int step = 2 * nwb, step_y = step / T, step_t = step - step_y * T;
+    const int step = NT * nwb, step_y = step / T, step_t = step - step_y * T;
     Pos pos{(bd.first + k) / T, (bd.first + k) - (bd.first + k) / T * T};
-    auto second = [&](Pos p, int kk) __attribute__((always_inline)) {
-        return kk + 1 < bd.count ? next_pos(p, 0, 1) : p;
+    // tile j of the group at p (the group's last tile when the band ends
+    // before it)
+    auto member = [&](Pos p, int kk, int j) __attribute__((always_inline)) {
+        return j > 0 && kk + j < bd.count ? next_pos(p, 0, j) : p;
     };
-    auto pair_src = [&](Pos p, int kk, const __bf16 *&sa, const __bf16 *&sb)
-                        __attribute__((always_inline)) {
-        const Pos q = second(p, kk);
-        const int xa = min(28 * p.t, nx - 28), xb = min(28 * q.t, nx - 28);
-        sa = a.in + ((size_t)p.y * Wp + xa + n) * PXS + coff;
-        sb = a.in + ((size_t)q.y * Wp + xb + n) * PXS + coff;
+    auto group_src = [&](Pos p, int kk, const __bf16 *(&src)[NT])
+                         __attribute__((always_inline)) {
+#pragma unroll
+        for (int j = 0; j < NT; j++) {
+            const Pos q = member(p, kk, j);
+            const int xq = min(28 * q.t, nx - 28);
+            src[j] = a.in + ((size_t)q.y * Wp + xq + n) * PXS + coff;
+        }
     };
-    const __bf16 *sa, *sb;
-    pair_src(pos, k, sa, sb);
-    f32x16 acc[2][5];
-    auto finish_pair = [&](Pos p, int kk) __attribute__((always_inline)) {
+    const __bf16 *src[NT];
+    group_src(pos, k, src);
+    f32x16 acc[NT][5];
+    auto finish_group = [&](Pos p, int kk) __attribute__((always_inline)) {
         finish(acc[0], p);
-        if (kk + 1 < bd.count) finish(acc[1], next_pos(p, 0, 1));  // wavefront-uniform
+        if constexpr (NT == 2) {
+            if (kk + 1 < bd.count) finish(acc[1], next_pos(p, 0, 1));  // wavefront-uniform
+        }
     };
 
-    // Two register slots of one kernel row each: row dy of a pair sits in
-    // slot (dy + P) & 1, and row 0 of the next pair is loaded with row 4
-    // of this one into slot (P + 1) & 1, so consecutive pairs alternate P.
-    bf16x8 buf[2][NC][2];
-    auto load_row = [&](const __bf16 *pa, const __bf16 *pb, int dy, int slot)
+    // Two register slots of one kernel row each: row dy of a group sits in
+    // slot (dy + P) & 1, and row 0 of the next group is loaded with row 4
+    // of this one into slot (P + 1) & 1, so consecutive groups alternate P.
+    bf16x8 buf[2][NC][NT];
+    auto load_row = [&](const __bf16 *const (&sp)[NT], int dy, int slot)
                         __attribute__((always_inline)) {
 #pragma unroll
         for (int c = 0; c < NC; c++) {
             const size_t o = (size_t)c * PL + (size_t)dy * RS;
-            buf[slot][c][0] = *reinterpret_cast<const bf16x8 *>(pa + o);
-            buf[slot][c][1] = *reinterpret_cast<const bf16x8 *>(pb + o);
+#pragma unroll
+            for (int j = 0; j < NT; j++)
+                buf[slot][c][j] = *reinterpret_cast<const bf16x8 *>(sp[j] + o);
         }
     };
     auto pair = [&](auto parity) __attribute__((always_inline)) {
         constexpr int P = decltype(parity)::value;
-        const bool more = k + 2 * nwb < bd.count;  // wavefront-uniform
-        const __bf16 *na = sa, *nb = sb;
+        const bool more = k + NT * nwb < bd.count;  // wavefront-uniform
+        const __bf16 *nsrc[NT];
+#pragma unroll
+        for (int j = 0; j < NT; j++) nsrc[j] = src[j];
         const Pos cur = pos;
         if (more) {
             pos = next_pos(pos, step_y, step_t);
-            pair_src(pos, k + 2 * nwb, na, nb);
+            group_src(pos, k + NT * nwb, nsrc);
         }
 #pragma unroll
-        for (int j = 0; j < 2; j++)
+        for (int j = 0; j < NT; j++)
 #pragma unroll
             for (int dx = 0; dx < 5; dx++) acc[j][dx] = f32x16{};
 #pragma unroll
         for (int dy = 0; dy < 5; dy++) {
+            // the last group loads its own row 0 again (unused): every path has
+            // the same loads outstanding, so the waits stay exact
             if (dy < 4)
-                load_row(sa, sb, dy + 1, (dy + 1 + P) & 1);
-            else if (more)
-                load_row(na, nb, 0, (P + 1) & 1);
+                load_row(src, dy + 1, (dy + 1 + P) & 1);
+            else
+                load_row(nsrc, 0, (P + 1) & 1);
             __builtin_amdgcn_sched_barrier(0);
             // channel-chunk-outer order: an accumulator comes back every
-            // 10 MFMAs, past the latency of the previous one
+            // 5 NT MFMAs, past the latency of the previous one
 #pragma unroll
             for (int c = 0; c < NC; c++)
 #pragma unroll
                 for (int dx = 0; dx < 5; dx++) {
                     const bf16x8 w = *reinterpret_cast<const bf16x8 *>(
                         wsh + ((dy * 5 + dx) * 32 + n) * ROW + h * 8 + c * 16);
-                    acc[0][dx] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                        w, buf[(dy + P) & 1][c][0], acc[0][dx], 0, 0, 0);
-                    acc[1][dx] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                        w, buf[(dy + P) & 1][c][1], acc[1][dx], 0, 0, 0);
+#pragma unroll
+                    for (int j = 0; j < NT; j++)
+                        acc[j][dx] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                            w, buf[(dy + P) & 1][c][j], acc[j][dx], 0, 0, 0);
                 }
         }
-        finish_pair(cur, k);
-        k += 2 * nwb;
-        sa = na;
-        sb = nb;
+        finish_group(cur, k);
+        k += NT * nwb;
+#pragma unroll
+        for (int j = 0; j < NT; j++) src[j] = nsrc[j];
     };
-    load_row(sa, sb, 0, 0);
+    load_row(src, 0, 0);
     while (true) {
         pair(std::integral_constant<int, 0>{});
         if (k >= bd.count) break;
@@ -316,6 +345,12 @@ __device__ __forceinline__ int in_tap(int c, int h, int q)
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_conv_in(
     ConvArgs a)
 {
+    // the bias sits in LDS: an LDS read in the epilogue waits on the LDS
+    // counter only, where a global load would wait for every store and
+    // every prefetched fragment before it
+    __shared__ __attribute__((aligned(16))) float bsh[HID];
+    if (threadIdx.x < HID) bsh[threadIdx.x] = a.bias[threadIdx.x];
+    __syncthreads();
     const int lane = (int)(threadIdx.x & 63), n = lane & 31, h = lane >> 5;
     const int nx = a.nx, Wp = nx + 4, T = (nx + 31) / 32;
     const size_t PL = plane_elems(nx, a.ny);
@@ -335,10 +370,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         wf[c] = __builtin_shufflevector(w[0], w[1], 0, 1, 2, 3, 4, 5, 6, 7);
     }
     const Band bd = xcd_band(a.ny * T);
-    const int wib = bd.local * 4 + (int)(threadIdx.x >> 6), nwb = bd.nlocal * 4;
+    // the wavefront's index is uniform: position arithmetic stays scalar
+    const int wib = __builtin_amdgcn_readfirstlane(bd.local * 4 + (int)(threadIdx.x >> 6));
+    const int nwb = bd.nlocal * 4;
     const int RS = Wp * IN0C;  // elements per padded input row
-    auto load = [&](int item, bf16x8(&b)[IN_CHUNKS]) __attribute__((always_inline)) {
-        const int y = item / T, t = item - y * T, x0 = min(32 * t, nx - 32);
+    struct Pos {
+        int y, t;
+    };
+    // item (row y, column tile t) = y * T + t, advanced by nwb items a trip
+    const int step_y = nwb / T, step_t = nwb - step_y * T;
+    auto advance = [&](Pos p) __attribute__((always_inline)) {
+        p.y += step_y;
+        p.t += step_t;
+        if (p.t >= T) {
+            p.t -= T;
+            p.y++;
+        }
+        return p;
+    };
+    auto load = [&](Pos q, bf16x8(&b)[IN_CHUNKS]) __attribute__((always_inline)) {
+        const int y = q.y, t = q.t, x0 = min(32 * t, nx - 32);
         // padded pixel (y, x0 + n): the tile's tap (0, 0) for this lane
         const __bf16 *px = a.in + ((size_t)y * Wp + x0 + n) * IN0C;
         const __bf16 *rows = px + 2 * h * IN0C;          // chunks 0..4: columns 2h, 2h + 1
@@ -353,22 +404,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         const bf16x4 r0 = *reinterpret_cast<const bf16x4 *>(col4);
         const bf16x4 r1 = *reinterpret_cast<const bf16x4 *>(col4 + RS);
         b[5] = __builtin_shufflevector(r0, r1, 0, 1, 2, 3, 4, 5, 6, 7);
+        // tap (4, 4) in both halves; half 1 zeroes it where the MFMA reads it,
+        // so that no VALU work waits for this load here
         const bf16x4 z = {};
         const bf16x4 t44 = *reinterpret_cast<const bf16x4 *>(px + 4 * RS + 4 * IN0C);
-        b[6] = __builtin_shufflevector(h ? z : t44, z, 0, 1, 2, 3, 4, 5, 6, 7);
+        b[6] = __builtin_shufflevector(t44, z, 0, 1, 2, 3, 4, 5, 6, 7);
     };
     int k = wib;
     if (k >= bd.count) return;
-    bf16x8 b[IN_CHUNKS], bn[IN_CHUNKS];
-    load(bd.first + k, b);
-    for (; k < bd.count; k += nwb) {
-        const int item = bd.first + k;
-        if (k + nwb < bd.count) load(item + nwb, bn);  // next item's fragments in flight
+    Pos pos{(bd.first + k) / T, (bd.first + k) - (bd.first + k) / T * T};
+    // two fragment sets: the next item's loads go to the other set while
+    // this item runs its MFMAs and epilogue (trips alternate P, no copies)
+    bf16x8 b[2][IN_CHUNKS];
+    load(pos, b[0]);
+    auto item = [&](auto parity) __attribute__((always_inline)) {
+        constexpr int P = decltype(parity)::value;
+        const Pos cur = pos;
+        // the last trip loads its own item again (unused): every path has the
+        // same loads outstanding, so the waits before the MFMAs name exactly
+        // this item's fragments
+        if (k + nwb < bd.count) pos = advance(pos);  // wavefront-uniform
+        load(pos, b[P ^ 1]);
         f32x16 acc = {};
 #pragma unroll
-        for (int c = 0; c < IN_CHUNKS; c++)
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wf[c], b[c], acc, 0, 0, 0);
-        const int y = item / T, t = item - y * T, x0 = min(32 * t, nx - 32);
+        for (int c = 0; c < IN_CHUNKS - 1; c++)
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wf[c], b[P][c], acc, 0, 0, 0);
+        const bf16x8 z8 = {};
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wf[IN_CHUNKS - 1],
+                                                      h ? z8 : b[P][IN_CHUNKS - 1], acc, 0, 0,
+                                                      0);
+        const int y = cur.y, t = cur.t, x0 = min(32 * t, nx - 32);
         const int x = x0 + n;
         if (x >= 32 * t) {
             const size_t px = pad_index(y, x, nx) * 16;
@@ -376,10 +441,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
             for (int g = 0; g < 4; g++) {
                 const int co = g * 8 + h * 4;
                 const size_t o = (size_t)(co >> 4) * PL + px + (co & 15);
+                const f32x4 bias = *reinterpret_cast<const f32x4 *>(&bsh[co]);
                 bf16x4 p, r;
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
-                    const float v = acc[4 * g + j] + a.bias[co + j];
+                    const float v = acc[4 * g + j] + bias[j];
                     p[j] = (__bf16)v;
                     r[j] = (__bf16)gelu(v);
                 }
@@ -387,8 +453,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
                 *reinterpret_cast<bf16x4 *>(a.out + o) = r;
             }
         }
-#pragma unroll
-        for (int c = 0; c < IN_CHUNKS; c++) b[c] = bn[c];
+        k += nwb;
+    };
+    while (true) {
+        item(std::integral_constant<int, 0>{});
+        if (k >= bd.count) break;
+        item(std::integral_constant<int, 1>{});
+        if (k >= bd.count) break;
     }
 }
 
@@ -1050,24 +1121,46 @@ dim3 conv_grid(int per_cu) { return dim3((unsigned)((cu_count() * per_cu + 7) / 
 
 }  // namespace
 
+namespace {
+// one tile per wavefront: 61.4 against 62.3 us per 32 -> 32 layer at 1024^2
+// on MI355X
+int g_conv_form = 1;
+}
+
+void set_conv_form(int form)
+{
+    if (form != 0 && form != 1)
+        NEMO_FATAL("closure conv: form %d (0: two tiles per wavefront, 1: one)", form);
+    g_conv_form = form;
+}
+
+int conv_form() { return g_conv_form; }
+
 void launch_conv(hipStream_t s, int cin, int cout, Epilogue epi, const ConvArgs &a)
 {
     if (a.nx < 32 || a.ny <= 0)
         NEMO_FATAL("closure conv: interior %d x %d (nx must be at least 32)", a.nx, a.ny);
-    // k_conv_sh: persistent, 2 workgroups per CU (LDS weights and 256
-    // registers); k_conv_in: persistent, 4 workgroups per CU (128
-    // registers); k_conv_out: one item per wavefront (out_grid)
-    const dim3 grid = conv_grid(2), block(256);
+    // k_conv_sh: persistent, 2 workgroups per CU (LDS weights; 4 wavefronts
+    // of 256 registers or 8 of 128); k_conv_in: persistent, 4 workgroups
+    // per CU (128 registers); k_conv_out: one item per wavefront (out_grid)
+    const dim3 grid = conv_grid(2), block(256), block8(512);
+    const bool one = g_conv_form == 1;
     if (cin == 16 && cout == 32 && epi == EPI_HIDDEN)  // the network input: 4 used channels
         k_conv_in<<<conv_grid(4), block, 0, s>>>(a);
+    else if (cin == 32 && cout == 32 && epi == EPI_HIDDEN && one)
+        k_conv_sh<2, EPI_HIDDEN, 1, 8><<<grid, block8, 0, s>>>(a);
     else if (cin == 32 && cout == 32 && epi == EPI_HIDDEN)
-        k_conv_sh<2, EPI_HIDDEN><<<grid, block, 0, s>>>(a);
+        k_conv_sh<2, EPI_HIDDEN, 2, 4><<<grid, block, 0, s>>>(a);
     else if (cin == 32 && cout == 16 && epi == EPI_OUTPUT)
         k_conv_out<<<out_grid(a.nx, a.ny), block, 0, s>>>(a);
+    else if (cin == 16 && cout == 32 && epi == EPI_DX && one)
+        k_conv_sh<1, EPI_DX, 1, 8><<<grid, block8, 0, s>>>(a);
     else if (cin == 16 && cout == 32 && epi == EPI_DX)
-        k_conv_sh<1, EPI_DX><<<grid, block, 0, s>>>(a);
+        k_conv_sh<1, EPI_DX, 2, 4><<<grid, block, 0, s>>>(a);
+    else if (cin == 32 && cout == 32 && epi == EPI_DX && one)
+        k_conv_sh<2, EPI_DX, 1, 8><<<grid, block8, 0, s>>>(a);
     else if (cin == 32 && cout == 32 && epi == EPI_DX)
-        k_conv_sh<2, EPI_DX><<<grid, block, 0, s>>>(a);
+        k_conv_sh<2, EPI_DX, 2, 4><<<grid, block, 0, s>>>(a);
     else
         NEMO_FATAL("closure conv: unsupported %d -> %d channels, epilogue %d", cin, cout,
                    (int)epi);

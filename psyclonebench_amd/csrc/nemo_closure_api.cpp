@@ -482,7 +482,9 @@ void nemo_cl_steps(void *h, void **ssha_t_device, int nsteps, int istep, int gua
 {
     ClosureContext &C = ctx(h);
     CorrectionHook user{&C, guard ? 1 : 0};
-    const StepHook hook{&correction_hook, &user, 2ull * C.generation + (guard ? 1 : 0)};
+    // the key names everything the captured kernels were built with
+    const StepHook hook{&correction_hook, &user,
+                        4ull * C.generation + 2ull * (unsigned)conv_form() + (guard ? 1 : 0)};
     psy_steps_with_hook(ssha_t_device, nsteps, istep, hook);
 }
 
@@ -496,6 +498,10 @@ int nemo_cl_guard_trips(void *h)
 }
 
 void nemo_cl_set_timing(void *h, int on) { ctx(h).timing = on != 0; }
+
+void nemo_cl_set_conv_form(int form) { set_conv_form(form); }
+
+int nemo_cl_conv_form(void) { return conv_form(); }
 
 double nemo_cl_forward_ms(void *h)
 {

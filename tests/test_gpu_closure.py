@@ -19,7 +19,8 @@ import oracle
 import oracle.closure_oracle as co
 from helpers import assert_bitwise
 from psyclonebench_amd import _native
-from psyclonebench_amd.closure import ClosureNet, coarse_model, init_params, layer_offsets
+from psyclonebench_amd.closure import (ClosureNet, coarse_model, conv_form, init_params,
+                                       layer_offsets, set_conv_form)
 from psyclonebench_amd.model import MASK_WEST_OPEN, Bathymetry, Model, NemoConfig, c1_config
 from psyclonebench_amd.psy import TimeStepper
 
@@ -68,6 +69,33 @@ def test_forward_matches_oracle():
     print("forward rel L2 error per channel:", errs, flush=True)
     assert max(errs) < FWD_TOL
     net.release()
+
+
+def test_conv_forms_bitwise_equal():
+    """Both launch forms of the 32 -> 32 convolutions (two tiles per
+    wavefront pass, or one) give the same bits: the forward output, the loss
+    and every gradient, on a 96 x 37 grid (the last 28-pixel column tile
+    overlaps its neighbour)."""
+    m = coarse_test_model(96, 37)
+    p = net_params(6)
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal((37, 96, 4))
+    t = rng.standard_normal((37, 96, 3))
+    out = {}
+    default = conv_form()
+    try:
+        for form in (0, 1):
+            set_conv_form(form)
+            net = ClosureNet(m, params=p)
+            y = net.forward_host(x)
+            loss, g = net.grad_host(x, t)
+            out[form] = (y, loss, g)
+            net.release()
+    finally:
+        set_conv_form(default)
+    assert_bitwise(out[1][0], out[0][0], "forward")
+    assert out[1][1] == out[0][1]
+    assert_bitwise(out[1][2], out[0][2], "gradients")
 
 
 @pytest.mark.parametrize("weights", [(1.0, 1.0, 1.0), (1.0, 3.0, 0.5)])
