@@ -136,14 +136,23 @@ void launch_set_fields(hipStream_t s, int mode, const double *sssh, const double
 // Face ssh from the T-cell ssh (next_sshu/v over their regions), when *flag.
 void launch_face_ssh(hipStream_t s, const int *flag, const CoarseState &c, int nx, int ny);
 // Energy guard: E = sum over interior cells of g*ssh^2 + ht*(u^2 + v^2),
-// before and after the correction, and a non-finite count, as per-workgroup
-// partial sums (partial[block][3]); a second launch sets
-// flag = finite && E_after <= (1 + rel) * E_before + abs and counts trips.
+// before and after the correction, and a non-finite count. The guarded output
+// layer (launch_conv_out_guarded) writes them as per-workgroup partial sums
+// (partial[block][3], guard_blocks(nx, ny) workgroups) beside the network
+// output; launch_guard_decide adds them in workgroup order and sets
+// flag = finite && E_after <= (1 + rel) * E_before + abs, counting trips.
 // A component beyond Norm::max_out counts as non-finite.
-int guard_blocks();
-void launch_guard(hipStream_t s, const float *pred, const Norm &nm, const CoarseState &c,
-                  double g, double *partial, double rel, double abs_tol, int *flag,
-                  int *trips, int nx, int ny);
+struct GuardSums {
+    Norm nm;
+    CoarseState c;
+    double g;
+    double *partial;
+};
+int guard_blocks(int nx, int ny);
+// the output layer (32 -> 16 channels, EPI_OUTPUT) with the guard's sums
+void launch_conv_out_guarded(hipStream_t s, const ConvArgs &a, const GuardSums &gs);
+void launch_guard_decide(hipStream_t s, const double *partial, int nblk, double rel,
+                         double abs_tol, int *flag, int *trips);
 
 }  // namespace cl
 }  // namespace nemo

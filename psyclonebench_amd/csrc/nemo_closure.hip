@@ -463,6 +463,62 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     }
 }
 
+// A wet T cell whose ssh the step does not clamp to the tide (bc_ssh sets
+// every wet cell with an open (-1) 4-neighbour).
+__device__ __forceinline__ bool free_ssh_cell(const int *tm, size_t k, int W)
+{
+    return tm[k] > 0 && tm[k - 1] >= 0 && tm[k + 1] >= 0 && tm[k - W] >= 0 && tm[k + W] >= 0;
+}
+
+// A cell whose ssh the closure corrects and learns: every wet cell
+// (ssh_wet), or the free cells only.
+__device__ __forceinline__ bool ssh_cell(const int *tm, size_t k, int W, int ssh_wet)
+{
+    return ssh_wet ? tm[k] > 0 : free_ssh_cell(tm, k, W);
+}
+
+// The coarse state at one interior point that the energy guard reads: ssh,
+// u, v, depth and the T mask of the point and its 4 neighbours.
+struct GuardPoint {
+    double s, u, v, h;
+    int tc, te, tw, tn, ts;
+};
+
+__device__ __forceinline__ GuardPoint guard_point(const CoarseState &c, size_t k)
+{
+    return GuardPoint{c.ssh[k],      c.u[k],         c.v[k],
+                      c.ht[k],       c.tmask[k],     c.tmask[k + 1],
+                      c.tmask[k - 1], c.tmask[k + c.W], c.tmask[k - c.W]};
+}
+
+// Energy-guard terms of interior point (I, J) whose network output is q:
+// E before and after the correction that k_set_fields applies there, and a
+// count of the non-finite or out-of-bound applied components.
+__device__ __forceinline__ void guard_terms(const float4 &q, const GuardSums &gs,
+                                            const GuardPoint &p, int I, int J, int nx, int ny,
+                                            double &e0, double &e1, double &bad)
+{
+    const Norm &nm = gs.nm;
+    const double g = gs.g;
+    const bool wet = p.tc > 0;
+    // ssh_cell() on the loaded mask values
+    const bool cs = nm.ssh_wet ? wet : wet && p.tw >= 0 && p.te >= 0 && p.ts >= 0 && p.tn >= 0;
+    const bool cu = I < nx && wet && p.te > 0;
+    const bool cv = J < ny && wet && p.tn > 0;
+    const double s = p.s, u = p.u, v = p.v, h = p.h;
+    const double s1 = cs ? s + (double)(q.z * nm.out_scale[2]) : s;
+    const double u1 = cu ? u + (double)(q.x * nm.out_scale[0]) : u;
+    const double v1 = cv ? v + (double)(q.y * nm.out_scale[1]) : v;
+    if (nm.max_out > 0.0f && ((cs && fabsf(q.z) > nm.max_out) ||
+                              (cu && fabsf(q.x) > nm.max_out) ||
+                              (cv && fabsf(q.y) > nm.max_out)))
+        bad += 1.0;
+    e0 += g * s * s + h * (u * u + v * v);
+    const double t = g * s1 * s1 + h * (u1 * u1 + v1 * v1);
+    if (isfinite(t)) e1 += t;
+    else bad += 1.0;
+}
+
 // Output layer (32 -> 3 used channels): the 16 rows of v_mfma_f32_16x16x32_bf16
 // hold (dx, co) = dx * 3 + co, so one instruction covers a whole kernel row
 // of 16 input pixels and 32 channels; the shifted sum over dx goes through a
@@ -471,19 +527,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
 // OUT_RB + 4 input rows once and keeps them in registers for the five kernel
 // rows that read them, and it keeps the five weight fragments in registers,
 // so no LDS fill and no workgroup barrier. One item per wavefront (the grid
-// covers every item: out_grid()).
+// covers every item: out_grid()). GUARD: the workgroup also sums the energy
+// guard's terms over the points it writes (guard_terms, in a fixed order:
+// lanes by a shuffle tree, then the 4 wavefronts) into gs.partial[block].
 constexpr int OUT_RB = 8;
 
-__global__ __launch_bounds__(256) void k_conv_out(ConvArgs a)
+template <bool GUARD>
+__global__ __launch_bounds__(256) void k_conv_out(ConvArgs a, GuardSums gs)
 {
     __shared__ float dsh[4][4][16][17];  // [wave][row][dx * 3 + co][pixel]
+    __shared__ double red[3][4];
     const int lane = (int)(threadIdx.x & 63), wv = (int)(threadIdx.x >> 6);
     const int n = lane & 15, q4 = lane >> 4, kg = q4 * 8;
     const int nx = a.nx, ny = a.ny, Wp = nx + 4, T = (nx + 11) / 12;
     const int nbands = (ny + OUT_RB - 1) / OUT_RB;
     const Band bd = xcd_band(nbands * T);
     const int k = bd.local * 4 + wv;
-    if (k >= bd.count) return;
+    double e0 = 0.0, e1 = 0.0, bad = 0.0;  // GUARD: this lane's terms
+    if (k < bd.count) {  // wavefront-uniform
     const int it = bd.first + k, band = it / T, t = it - band * T;
     const int y0 = band * OUT_RB, nr = min(OUT_RB, ny - y0), x0 = min(12 * t, nx - 12);
     const size_t PL = plane_elems(nx, ny), RS = (size_t)Wp * 16;
@@ -503,6 +564,19 @@ __global__ __launch_bounds__(256) void k_conv_out(ConvArgs a)
 #pragma unroll
     for (int r = 0; r < OUT_RB + 4; r++)
         rows[r] = r < nr + 4 ? *reinterpret_cast<const bf16x8 *>(src + r * RS) : bf16x8{};
+    // GUARD: the state at this lane's output points (row 4 g4 + q4 of the
+    // band), loaded with the input rows so that their latency hides behind
+    // the MFMAs; lanes without a point read interior point (1, 1)
+    GuardPoint gp[OUT_RB / 4];
+    if constexpr (GUARD) {
+#pragma unroll
+        for (int g4 = 0; g4 < OUT_RB / 4; g4++) {
+            const int yy = 4 * g4 + q4, x = x0 + n;
+            const bool mine = yy < nr && n < 12 && x >= 12 * t;
+            gp[g4] = guard_point(gs.c, mine ? (size_t)(y0 + yy + 1) * gs.c.W + (x + 1)
+                                            : (size_t)gs.c.W + 1);
+        }
+    }
 
     float(*d)[16][17] = dsh[wv];
     const int nc = min(n, 11);
@@ -544,6 +618,26 @@ __global__ __launch_bounds__(256) void k_conv_out(ConvArgs a)
             r.z = v[2] * gt;
             r.w = 0.0f;
             reinterpret_cast<float4 *>(a.outf)[p] = r;
+            if constexpr (GUARD)
+                guard_terms(r, gs, gp[g4], x + 1, y0 + yy + 1, nx, ny, e0, e1, bad);
+        }
+    }
+    }
+    if constexpr (GUARD) {
+        for (int off = 32; off > 0; off >>= 1) {
+            e0 += __shfl_xor(e0, off);
+            e1 += __shfl_xor(e1, off);
+            bad += __shfl_xor(bad, off);
+        }
+        if (lane == 0) {
+            red[0][wv] = e0;
+            red[1][wv] = e1;
+            red[2][wv] = bad;
+        }
+        __syncthreads();
+        if (threadIdx.x < 3) {
+            const int r = (int)threadIdx.x;
+            gs.partial[blockIdx.x * 3 + r] = (red[r][0] + red[r][1]) + (red[r][2] + red[r][3]);
         }
     }
 }
@@ -892,21 +986,6 @@ __global__ void k_target(const double *__restrict__ fssh, const double *__restri
     reinterpret_cast<float4 *>(target)[p] = t;
 }
 
-// A wet T cell whose ssh the step does not clamp to the tide (bc_ssh sets
-// every wet cell with an open (-1) 4-neighbour).
-constexpr int GUARD_PPT = 4;  // guard points per thread and pass
-
-__device__ __forceinline__ bool free_ssh_cell(const int *tm, size_t k, int W)
-{
-    return tm[k] > 0 && tm[k - 1] >= 0 && tm[k + 1] >= 0 && tm[k - W] >= 0 && tm[k + W] >= 0;
-}
-
-// A cell whose ssh the closure corrects and learns: every wet cell
-// (ssh_wet), or the free cells only.
-__device__ __forceinline__ bool ssh_cell(const int *tm, size_t k, int W, int ssh_wet)
-{
-    return ssh_wet ? tm[k] > 0 : free_ssh_cell(tm, k, W);
-}
 
 __global__ __launch_bounds__(256) void k_loss(const float *__restrict__ pred,
                                               const float *__restrict__ target,
@@ -1016,62 +1095,6 @@ __global__ void k_face_ssh(const int *__restrict__ flag, CoarseState c, int nx, 
         c.sshv[k] = val;
 }
 
-__global__ __launch_bounds__(256) void k_guard_partial(const float *__restrict__ pred,
-                                                       Norm nm, CoarseState c, double g,
-                                                       double *__restrict__ partial,
-                                                       int nx, int ny)
-{
-    // GUARD_PPT points per thread per pass, all loads of a pass in flight
-    // together; wavefront sums by shuffles, then one LDS step over the 4
-    // wavefronts (a fixed order: the same sums every call).
-    __shared__ double red[3][4];
-    double e0 = 0.0, e1 = 0.0, bad = 0.0;
-    const int npx = nx * ny, stride = (int)(gridDim.x * blockDim.x);
-    for (int p0 = (int)(blockIdx.x * blockDim.x + threadIdx.x); p0 < npx;
-         p0 += GUARD_PPT * stride) {
-#pragma unroll
-        for (int j = 0; j < GUARD_PPT; j++) {
-            const int p = p0 + j * stride;
-            if (p >= npx) break;
-            const int J = p / nx + 1, I = p - (J - 1) * nx + 1;
-            const size_t k = (size_t)J * c.W + I;
-            const float4 q = reinterpret_cast<const float4 *>(pred)[p];
-            const double s = c.ssh[k], u = c.u[k], v = c.v[k], h = c.ht[k];
-            const bool wet = c.tmask[k] > 0;
-            const bool cs = ssh_cell(c.tmask, k, c.W, nm.ssh_wet);
-            const bool cu = I < nx && wet && c.tmask[k + 1] > 0;
-            const bool cv = J < ny && wet && c.tmask[k + c.W] > 0;
-            const double s1 = cs ? s + (double)(q.z * nm.out_scale[2]) : s;
-            const double u1 = cu ? u + (double)(q.x * nm.out_scale[0]) : u;
-            const double v1 = cv ? v + (double)(q.y * nm.out_scale[1]) : v;
-            if (nm.max_out > 0.0f && ((cs && fabsf(q.z) > nm.max_out) ||
-                                      (cu && fabsf(q.x) > nm.max_out) ||
-                                      (cv && fabsf(q.y) > nm.max_out)))
-                bad += 1.0;
-            e0 += g * s * s + h * (u * u + v * v);
-            const double t = g * s1 * s1 + h * (u1 * u1 + v1 * v1);
-            if (isfinite(t)) e1 += t;
-            else bad += 1.0;
-        }
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-        e0 += __shfl_xor(e0, off);
-        e1 += __shfl_xor(e1, off);
-        bad += __shfl_xor(bad, off);
-    }
-    const int lane = (int)(threadIdx.x & 63), wv = (int)(threadIdx.x >> 6);
-    if (lane == 0) {
-        red[0][wv] = e0;
-        red[1][wv] = e1;
-        red[2][wv] = bad;
-    }
-    __syncthreads();
-    if (threadIdx.x < 3) {
-        const int r = (int)threadIdx.x;
-        partial[blockIdx.x * 3 + r] = (red[r][0] + red[r][1]) + (red[r][2] + red[r][3]);
-    }
-}
-
 // Adds the partial sums in workgroup order (the same sum every call) and
 // decides. A separate launch rather than a last-workgroup epilogue: the
 // device-scope fence that epilogue needs writes back each XCD's L2.
@@ -1152,7 +1175,7 @@ void launch_conv(hipStream_t s, int cin, int cout, Epilogue epi, const ConvArgs 
     else if (cin == 32 && cout == 32 && epi == EPI_HIDDEN)
         k_conv_sh<2, EPI_HIDDEN, 2, 4><<<grid, block, 0, s>>>(a);
     else if (cin == 32 && cout == 16 && epi == EPI_OUTPUT)
-        k_conv_out<<<out_grid(a.nx, a.ny), block, 0, s>>>(a);
+        k_conv_out<false><<<out_grid(a.nx, a.ny), block, 0, s>>>(a, GuardSums{});
     else if (cin == 16 && cout == 32 && epi == EPI_DX && one)
         k_conv_sh<1, EPI_DX, 1, 8><<<grid, block8, 0, s>>>(a);
     else if (cin == 16 && cout == 32 && epi == EPI_DX)
@@ -1291,19 +1314,21 @@ void launch_face_ssh(hipStream_t s, const int *flag, const CoarseState &c, int n
     NEMO_LAUNCH_CHECK();
 }
 
-// GUARD_PPT (4) points per thread cover a 1024 x 1024 coarse grid in one
-// pass of 1024 workgroups
-int guard_blocks() { return 1024; }
+// one partial per workgroup of the output layer
+int guard_blocks(int nx, int ny) { return (int)out_grid(nx, ny).x; }
 
-void launch_guard(hipStream_t s, const float *pred, const Norm &nm, const CoarseState &c,
-                  double g, double *partial, double rel, double abs_tol, int *flag,
-                  int *trips, int nx, int ny)
+void launch_conv_out_guarded(hipStream_t s, const ConvArgs &a, const GuardSums &gs)
 {
-    k_guard_partial<<<dim3((unsigned)guard_blocks()), dim3(256), 0, s>>>(pred, nm, c, g,
-                                                                         partial, nx, ny);
+    if (a.nx < 32 || a.ny <= 0)
+        NEMO_FATAL("closure conv: interior %d x %d (nx must be at least 32)", a.nx, a.ny);
+    k_conv_out<true><<<out_grid(a.nx, a.ny), dim3(256), 0, s>>>(a, gs);
     NEMO_LAUNCH_CHECK();
-    k_guard_decide<<<dim3(1), dim3(256), 0, s>>>(partial, guard_blocks(), rel, abs_tol, flag,
-                                                 trips);
+}
+
+void launch_guard_decide(hipStream_t s, const double *partial, int nblk, double rel,
+                         double abs_tol, int *flag, int *trips)
+{
+    k_guard_decide<<<dim3(1), dim3(256), 0, s>>>(partial, nblk, rel, abs_tol, flag, trips);
     NEMO_LAUNCH_CHECK();
 }
 

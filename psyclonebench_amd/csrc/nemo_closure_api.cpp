@@ -107,7 +107,9 @@ CoarseState state(ClosureContext &C, double *ssh, double *u, double *v, double *
 // Forward pass from C.in0; training keeps the pre-activations; gated: the
 // output is multiplied by the activity gate k_pack computed with the input;
 // timed: HIP events around it (C.timing, outside captured graphs).
-void forward(ClosureContext &C, hipStream_t s, bool training, bool gated, bool timed)
+// guard: the output layer also writes the energy guard's partial sums
+void forward(ClosureContext &C, hipStream_t s, bool training, bool gated, bool timed,
+             const GuardSums *guard = nullptr)
 {
     if (timed) NEMO_HIP_CHECK(hipEventRecord(C.ev0, s));
     const __bf16 *x = C.in0;
@@ -126,7 +128,10 @@ void forward(ClosureContext &C, hipStream_t s, bool training, bool gated, bool t
         } else {
             a.outf = C.pred;
             a.gate = gated ? C.gate : nullptr;
-            launch_conv(s, layer_cin(l), layer_cout(l), EPI_OUTPUT, a);
+            if (guard)
+                launch_conv_out_guarded(s, a, *guard);
+            else
+                launch_conv(s, layer_cin(l), layer_cout(l), EPI_OUTPUT, a);
         }
     }
     if (timed) NEMO_HIP_CHECK(hipEventRecord(C.ev1, s));
@@ -138,11 +143,11 @@ void enqueue_correction(ClosureContext &C, hipStream_t s, int guard, const Coars
                         bool timed)
 {
     launch_pack(s, c, C.nm, C.in0, C.gate, C.nx, C.ny);
-    forward(C, s, false, true, timed);
-    if (guard) {
-        launch_guard(s, C.pred, C.nm, c, C.g, C.guard_partial, C.guard_rel, C.guard_abs,
-                     C.flag, C.trips, C.nx, C.ny);
-    }
+    const GuardSums gs{C.nm, c, C.g, C.guard_partial};
+    forward(C, s, false, true, timed, guard ? &gs : nullptr);
+    if (guard)
+        launch_guard_decide(s, C.guard_partial, guard_blocks(C.nx, C.ny), C.guard_rel,
+                            C.guard_abs, C.flag, C.trips);
     const int *flag = guard ? C.flag : nullptr;
     launch_set_fields(s, 1, nullptr, nullptr, nullptr, C.pred, C.nm, flag, c, C.nx, C.ny);
     launch_face_ssh(s, flag, c, C.nx, C.ny);
@@ -283,7 +288,7 @@ void *nemo_cl_create(int nx, int ny, int coarse_width, int coarse_height,
     C->fssh = zeros<double>(nc, s);
     C->fu = zeros<double>(nc, s);
     C->fv = zeros<double>(nc, s);
-    C->guard_partial = zeros<double>((size_t)guard_blocks() * 3, s);
+    C->guard_partial = zeros<double>((size_t)guard_blocks(nx, ny) * 3, s);
     C->flag = zeros<int>(1, s);
     C->trips = zeros<int>(1, s);
     NEMO_HIP_CHECK(hipEventCreate(&C->ev0));
@@ -448,7 +453,8 @@ void nemo_cl_last_stats(void *h, double *out)
     ClosureContext &C = ctx(h);
     std::vector<float> p((size_t)loss_blocks() * 3);
     float ns[2];
-    std::vector<double> gp((size_t)guard_blocks() * 3);
+    const int gblk = guard_blocks(C.nx, C.ny);
+    std::vector<double> gp((size_t)gblk * 3);
     NEMO_HIP_CHECK(hipStreamSynchronize(lib_stream()));
     NEMO_HIP_CHECK(hipMemcpy(p.data(), C.loss_partial, p.size() * sizeof(float),
                              hipMemcpyDeviceToHost));
@@ -464,7 +470,7 @@ void nemo_cl_last_stats(void *h, double *out)
     out[3] = ns[0];
     for (int r = 0; r < 3; r++) {
         double s = 0.0;
-        for (int b = 0; b < guard_blocks(); b++) s += gp[(size_t)b * 3 + r];
+        for (int b = 0; b < gblk; b++) s += gp[(size_t)b * 3 + r];
         out[4 + r] = s;
     }
 }

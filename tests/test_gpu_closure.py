@@ -253,6 +253,42 @@ def test_guard_keeps_plain_step_on_blowup():
     net.release()
 
 
+def test_guard_sums_equal_energy_of_the_state():
+    """The energy sums the output layer folds in (E before and after the
+    correction, the non-finite count) against numpy over the interior of
+    the coarse state fetched before and after an accepted correction:
+    E = sum g ssh^2 + ht (u^2 + v^2), to 1e-12 relative (the GPU sums in
+    another order)."""
+    _, coarse = _fine_and_coarse()
+    sc = TimeStepper(coarse, mode=2)
+    sc.run(5)
+    g = coarse.grid
+    nx, ny = g.M - 2, g.N - 2
+    net = ClosureNet(coarse, params=net_params(7))
+    net.set_norm([0, 0, 0, 100], [1, 1, 1, 50], [1e-4, 1e-4, 1e-4])
+    net.set_guard(1e6, 1e6)  # accept
+    grav = 9.80665
+    ht = coarse.array("ht")[1:ny + 1, 1:nx + 1]
+
+    def energy(st):
+        s_, u_, v_ = (st[n][1:ny + 1, 1:nx + 1] for n in ("sshn_t", "un", "vn"))
+        return float(np.sum(grav * s_ * s_ + ht * (u_ * u_ + v_ * v_)))
+
+    before = {n: sc.fetch(n) for n in ("sshn_t", "un", "vn")}
+    net.correct(sc, guard=True)
+    after = {n: sc.fetch(n) for n in ("sshn_t", "un", "vn")}
+    assert net.guard_trips() == 0
+    assert any(np.any(after[n] != before[n]) for n in after)
+    st = net.last_stats()
+    e0, e1 = energy(before), energy(after)
+    print("guard energies", st["energy_before"], e0, st["energy_after"], e1, flush=True)
+    assert st["nonfinite"] == 0.0
+    assert abs(st["energy_before"] - e0) <= 1e-12 * abs(e0)
+    assert abs(st["energy_after"] - e1) <= 1e-12 * abs(e1)
+    assert e1 != e0
+    net.release()
+
+
 @pytest.mark.timeout(600)
 def test_shipped_weights_halve_the_plain_coarse_error():
     """The bar the learned closure is built to: on held-out bathymetries
