@@ -1,7 +1,7 @@
 """Training and evaluation of the learned coarse closure on the GPU.
 
     python -m psyclonebench_amd.closure_run train --n 4096 --bathy 0:64 --steps 500 \\
-        --out psyclonebench_amd/data/closure_weights.npz
+        --seed 1 --out psyclonebench_amd/data/closure_weights.npz
     python -m psyclonebench_amd.closure_run eval --n 4096 --bathy 64:80 --steps 2000 \\
         --weights psyclonebench_amd/data/closure_weights.npz
 
@@ -465,11 +465,11 @@ def main(argv=None) -> None:
     t.add_argument("--bathy", default="0:64")
     t.add_argument("--steps", type=int, default=500)
     # defaults: the recipe of the shipped weights
-    t.add_argument("--lr", type=float, default=1e-3)
-    t.add_argument("--lr-min", type=float, default=1e-4)
+    t.add_argument("--lr", type=float, default=3e-4)
+    t.add_argument("--lr-min", type=float, default=3e-5)
     t.add_argument("--seed", type=int, default=0)
     t.add_argument("--log-every", type=int, default=100)
-    t.add_argument("--rollout", type=int, default=30,
+    t.add_argument("--rollout", type=int, default=15,
                    help="longest stretch of steps on the corrected state between resets")
     t.add_argument("--guard-rel", type=float, default=0.25)
     t.add_argument("--clip", type=float, default=1.0,
