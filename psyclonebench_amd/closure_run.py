@@ -456,7 +456,7 @@ def throughput(n: int, factor: int, weights, steps: int = 200) -> dict:
             "speedup_vs_fine": fine_ms / learned_ms}
 
 
-def main(argv=None) -> None:
+def build_parser() -> argparse.ArgumentParser:
     p = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     sub = p.add_subparsers(dest="cmd", required=True)
     t = sub.add_parser("train")
@@ -534,7 +534,11 @@ def main(argv=None) -> None:
     s.add_argument("--conv-form", type=int, default=None,
                    help="launch form of the 32 -> 32 convolutions (0 or 1; default: the "
                         "library's)")
-    a = p.parse_args(argv)
+    return p
+
+
+def main(argv=None) -> None:
+    a = build_parser().parse_args(argv)
     if a.cmd == "train":
         train(a)
     elif a.cmd == "eval":
