@@ -1098,18 +1098,34 @@ __global__ void k_face_ssh(const int *__restrict__ flag, CoarseState c, int nx, 
 // Adds the partial sums in workgroup order (the same sum every call) and
 // decides. A separate launch rather than a last-workgroup epilogue: the
 // device-scope fence that epilogue needs writes back each XCD's L2.
-__global__ __launch_bounds__(256) void k_guard_decide(const double *__restrict__ partial,
-                                                      int nblk, double rel, double abs_tol,
-                                                      int *__restrict__ flag,
-                                                      int *__restrict__ trips)
+// One workgroup of DECIDE_THREADS threads; each thread adds the partials
+// b = tid, tid + DECIDE_THREADS, ... with DECIDE_UNROLL of them loaded
+// together (one memory latency for a few thousand partials), then a tree in
+// LDS.
+constexpr int DECIDE_THREADS = 1024, DECIDE_UNROLL = 4;
+
+__global__ __launch_bounds__(DECIDE_THREADS) void k_guard_decide(
+    const double *__restrict__ partial, int nblk, double rel, double abs_tol,
+    int *__restrict__ flag, int *__restrict__ trips)
 {
-    __shared__ double red[3][256];
+    __shared__ double red[3][DECIDE_THREADS];
     double a[3] = {0.0, 0.0, 0.0};
-    for (int b = (int)threadIdx.x; b < nblk; b += 256)
-        for (int r = 0; r < 3; r++) a[r] += partial[b * 3 + r];
+    for (int b0 = (int)threadIdx.x; b0 < nblk; b0 += DECIDE_UNROLL * DECIDE_THREADS) {
+        double v[DECIDE_UNROLL][3];
+#pragma unroll
+        for (int u = 0; u < DECIDE_UNROLL; u++) {
+            const int b = b0 + u * DECIDE_THREADS;
+#pragma unroll
+            for (int r = 0; r < 3; r++) v[u][r] = b < nblk ? partial[b * 3 + r] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < DECIDE_UNROLL; u++)
+#pragma unroll
+            for (int r = 0; r < 3; r++) a[r] += v[u][r];
+    }
     for (int r = 0; r < 3; r++) red[r][threadIdx.x] = a[r];
     __syncthreads();
-    for (int k = 128; k > 0; k >>= 1) {
+    for (int k = DECIDE_THREADS / 2; k > 0; k >>= 1) {
         if ((int)threadIdx.x < k)
             for (int r = 0; r < 3; r++) red[r][threadIdx.x] += red[r][threadIdx.x + k];
         __syncthreads();
@@ -1328,7 +1344,8 @@ void launch_conv_out_guarded(hipStream_t s, const ConvArgs &a, const GuardSums &
 void launch_guard_decide(hipStream_t s, const double *partial, int nblk, double rel,
                          double abs_tol, int *flag, int *trips)
 {
-    k_guard_decide<<<dim3(1), dim3(256), 0, s>>>(partial, nblk, rel, abs_tol, flag, trips);
+    k_guard_decide<<<dim3(1), dim3(DECIDE_THREADS), 0, s>>>(partial, nblk, rel, abs_tol, flag,
+                                                            trips);
     NEMO_LAUNCH_CHECK();
 }
 
