@@ -394,8 +394,8 @@ def test_full_size_4096_live_state_unfused_equals_oracle():
 def test_c2_workload_1000_steps_equals_oracle():
     """The 4096^2 FP64 workload itself (seeded bathymetry, open west edge,
     1 m tide) for its full 1000 steps as one graph replay, against 1000
-    OpenMP oracle steps: bitwise on every output. The north-star bound is
-    1e-12 relative; the bar met is equality."""
+    OpenMP oracle steps: bitwise on every output. The stated bound for this
+    workload is 1e-12 relative; the bar met is equality."""
     cfg = c2_config(4096, 1000, seed=0)
     m = Model(cfg)
     g = m.grid
