@@ -247,7 +247,8 @@ NEMO_API void *nemo_ens_create(int ndom, void *const *arrays, const int *geom,
  * or 0: per domain, whichever of the two its width runs faster with) and
  * nemo_ens_shape_prefetch the rows of loads the tile issues ahead of the row
  * that needs them, beyond the next one (0, 1, or 2 with one workgroup per
- * tile). */
+ * tile). A new ensemble runs one wavefront per 4-row tile, per-domain
+ * columns, prefetch 2, until nemo_ens_set_shape or nemo_ens_autotune. */
 NEMO_API int nemo_ens_shape_count(void *handle);
 NEMO_API void nemo_ens_shape_info(void *handle, int shape, int *rows,
                                   int *waves_per_block, int *blocks);
